@@ -1,0 +1,67 @@
+"""The CPU oracle against the reference's own outputs (golden fixtures) and
+against closed forms.  A mismatch here means the oracle does not restate the
+reference, so every GPU parity test built on it would be void."""
+import math
+
+import numpy as np
+import pytest
+
+import golden_cases
+from oracle import orc
+
+
+@pytest.fixture(scope="module", params=golden_cases.CASES)
+def case_run(request):
+    c = golden_cases.load(request.param)
+    spec = c["spec"]
+    o = orc.Oracle(spec, c["medium"])
+    mon = spec["monitor"]
+    out = spec["output"]
+    rows, slices, full = [], {}, []
+    for t in range(spec["grid"]["tsteps"]):
+        o.step(1)
+        rows.append(o.monitor_row(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"]))
+        if t % out["interval"] == 0:
+            slices[t] = o.slice(out["component"], out["axis"], out["pos"])
+            fr = list(o.monitor_row(mon["x"], mon["y"], mon["z"])[:10])
+            r, v = o.cell_ions(mon["x"], mon["y"], mon["z"])
+            for a in range(len(r) // 3):
+                fr += list(r[3 * a:3 * a + 3]) + list(v[3 * a:3 * a + 3])
+            full.append(fr)
+    c["oracle_full"] = np.array(full)
+    return c, o, np.array(rows), slices
+
+
+def test_monitor_trace_bit_exact(case_run):
+    c, o, rows, _ = case_run
+    ref = c["monitor"]
+    ncol = ref.shape[1]
+    assert ref.shape[0] == rows.shape[0]
+    got = rows[:, :ncol]
+    # the oracle restates the reference's FP64 operation order: bit-exact
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_full_monitor_bit_exact(case_run):
+    c, o, rows, _ = case_run
+    # every output step: 9 fields at z+1 and r, v of all atoms (fdtd.c:3729-3779)
+    np.testing.assert_array_equal(c["oracle_full"], c["full"])
+
+
+def test_slices_exact(case_run):
+    c, o, rows, slices = case_run
+    assert sorted(slices) == sorted(c["slices"])
+    for t, s in slices.items():
+        np.testing.assert_array_equal(s, c["slices"][t], err_msg="slice at step %d" % t)
+
+
+def test_signal_present(case_run):
+    c, o, rows, slices = case_run
+    spec = c["spec"]
+    # guard against a vacuous pass: the samples carry signal ...
+    assert max(np.abs(s).max() for s in slices.values()) > 0
+    if spec["monitor"]["z"] == spec["grid"]["nz"] - 1:
+        # ... except a monitor on the top plane, which reads the zero halo (fdtd.c:3814)
+        assert np.abs(rows[:, 1:10]).max() == 0
+    else:
+        assert np.abs(rows[:, 1:10]).max() > 0
