@@ -215,6 +215,10 @@ int64_t phpfdtd_step(const phpfdtd_ctx *ctx);
 /* Message of the last failure on this thread ("" if none). */
 const char *phpfdtd_last_error(void);
 
+/* sizeof of phpfdtd_grid, _material, _source, _diag, _ktime as compiled into
+ * the library (binding checks); returns the number of sizes known (5). */
+int  phpfdtd_struct_sizes(int *sizes, int n);
+
 int  phpfdtd_abi_version(void);
 void phpfdtd_destroy(phpfdtd_ctx *ctx);
 

@@ -1,0 +1,183 @@
+// kernels_aux.hip - per-step epilogue, diagnostics and sampling kernels.
+#include "phpfdtd_internal.h"
+
+namespace phpfdtd {
+
+// Per-step epilogue (one thread): the monitor row of fdtd.c:3806-3851 for the
+// step that just finished, then the device step counter advances.  The row is
+// written at index `step`, so graph replays need no host bookkeeping.
+template <typename real>
+__global__ void step_end(StepEndArgs a) {
+    if (threadIdx.x != 0) return;
+    const int64_t st = *a.step;
+    if (a.mon_on && st < a.ring_rows) {
+        double *row = a.ring + st * 12;
+        row[0] = (double)st * a.dt;
+        for (int q = 0; q < 9; ++q)
+            row[1 + q] = a.mon_c >= 0 ? (double)static_cast<const real *>(a.F[q])[a.mon_c] : 0.0;
+        if (a.mon_dof >= 0) {
+            row[10] = a.mon_r[a.mon_dof * a.mon_nslot + a.mon_slot];
+            row[11] = a.mon_v[a.mon_dof * a.mon_nslot + a.mon_slot];
+        } else {
+            row[10] = 0.0;
+            row[11] = 0.0;
+        }
+    }
+    *a.step = st + 1;
+}
+
+template <typename T>
+__global__ void fill(T *p, T v, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        p[q] = v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = v > o ? v : o;
+    }
+    return v;
+}
+
+// max |E| and max |Ex| over the physical cells (fdtd.c:3537-3552).  Values are
+// non-negative, so their IEEE bit patterns order like the values.
+template <typename real>
+__global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, const real *Ez, int nx, int ny,
+                                                int nz, int nxp, unsigned long long *out) {
+    const int64_t total = (int64_t)nxp * ny * nz;
+    unsigned long long me = 0, mx = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if ((int)(c % nxp) >= nx) continue;
+        const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+        const double e = sqrt(ex * ex + ey * ey + ez * ez);
+        const unsigned long long be = __double_as_longlong(e), bx = __double_as_longlong(fabs(ex));
+        me = me > be ? me : be;
+        mx = mx > bx ? mx : bx;
+    }
+    me = wave_max_u64(me);
+    mx = wave_max_u64(mx);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out + 0, me);
+        atomicMax(out + 1, mx);
+    }
+}
+
+// max |r| over one lattice material's state.
+__global__ __launch_bounds__(256) void reduce_r(const double *r, int64_t n, unsigned long long *out) {
+    unsigned long long m = 0;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long b = __double_as_longlong(fabs(r[q]));
+        m = m > b ? m : b;
+    }
+    m = wave_max_u64(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(out + 2, m);
+}
+
+// First argmax in the reference's loop order (i, j, k, atom, dir): among the
+// dofs equal to the maximum, the smallest key ((i*ny + j)*nz + k)*96 + q.
+__global__ __launch_bounds__(256) void argmax_r(const double *r, const uint32_t *cells, int64_t nslot, int n3,
+                                                int ny, int nz, int nxp, unsigned long long *out) {
+    const unsigned long long target = out[2];
+    const int64_t total = nslot * n3;
+    unsigned long long best = ~0ull;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        if ((unsigned long long)__double_as_longlong(fabs(r[q])) != target) continue;
+        const int64_t dof = q / nslot, s = q % nslot;
+        const int64_t c = cells[s];
+        const int64_t i = c % nxp, j = (c / nxp) % ny, k = c / ((int64_t)nxp * ny);
+        const unsigned long long key = (unsigned long long)(((i * ny + j) * nz + k) * 96 + dof);
+        best = best < key ? best : key;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(best, off, 64);
+        best = best < o ? best : o;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMin(out + 3, best);
+}
+
+// One plane as float32 in the reference's slice layout (fdtd.c:999-1400).
+// comp < 0 selects |E| computed in FP64 from Ex, Ey, Ez (fdtd.c:3693-3703).
+template <typename real>
+__global__ __launch_bounds__(256) void slice_k(const real *F, const real *Ex, const real *Ey, const real *Ez,
+                                               int emag, char axis, int pos, int nx, int ny, int nz, int nxp,
+                                               float *out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c;
+    if (axis == 'z') {
+        if (q >= (int64_t)nx * ny) return;
+        const int i = (int)(q / ny), j = (int)(q % ny);
+        c = ((int64_t)pos * ny + j) * nxp + i;
+    } else if (axis == 'x') {
+        if (q >= (int64_t)ny * nz) return;
+        const int j = (int)(q / nz), k = (int)(q % nz);
+        c = ((int64_t)k * ny + j) * nxp + pos;
+    } else {
+        if (q >= (int64_t)nx * nz) return;
+        const int i = (int)(q / nz), k = (int)(q % nz);
+        c = ((int64_t)k * ny + pos) * nxp + i;
+    }
+    double v;
+    if (emag) {
+        const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+        v = sqrt(ex * ex + ey * ey + ez * ez);
+    } else {
+        v = (double)F[c];
+    }
+    out[q] = (float)v;
+}
+
+template <typename real>
+void launch_step_end(const StepEndArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(step_end<real>, dim3(1), dim3(64), 0, s, a);
+}
+
+template <typename T>
+void launch_fill(T *p, T v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(fill<T>, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(256), 0, s, p, v, n);
+}
+
+template <typename real>
+void launch_reduce_e(const real *Ex, const real *Ey, const real *Ez, int nx, int ny, int nz, int nxp,
+                     unsigned long long *out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_e<real>, dim3(2048), dim3(256), 0, s, Ex, Ey, Ez, nx, ny, nz, nxp, out);
+}
+
+void launch_reduce_r(const double *r, int64_t n, unsigned long long *out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(reduce_r, dim3(1024), dim3(256), 0, s, r, n, out);
+}
+
+void launch_argmax_r(const double *r, const uint32_t *cells, int64_t nslot, int n3, int ny, int nz, int nxp,
+                     unsigned long long *out, hipStream_t s) {
+    if (nslot > 0) hipLaunchKernelGGL(argmax_r, dim3(1024), dim3(256), 0, s, r, cells, nslot, n3, ny, nz, nxp, out);
+}
+
+template <typename real>
+void launch_slice(const real *F, const real *Ex, const real *Ey, const real *Ez, int emag, char axis, int pos,
+                  int nx, int ny, int nz, int nxp, float *out, hipStream_t s) {
+    int64_t n = axis == 'z' ? (int64_t)nx * ny : axis == 'x' ? (int64_t)ny * nz : (int64_t)nx * nz;
+    hipLaunchKernelGGL(slice_k<real>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, F, Ex, Ey, Ez, emag,
+                       axis, pos, nx, ny, nz, nxp, out);
+}
+
+template void launch_step_end<double>(const StepEndArgs &, hipStream_t);
+template void launch_step_end<float>(const StepEndArgs &, hipStream_t);
+template void launch_fill<double>(double *, double, int64_t, hipStream_t);
+template void launch_fill<float>(float *, float, int64_t, hipStream_t);
+template void launch_fill<unsigned long long>(unsigned long long *, unsigned long long, int64_t, hipStream_t);
+template void launch_fill<int64_t>(int64_t *, int64_t, int64_t, hipStream_t);
+template void launch_fill<uint8_t>(uint8_t *, uint8_t, int64_t, hipStream_t);
+template void launch_reduce_e<double>(const double *, const double *, const double *, int, int, int, int,
+                                      unsigned long long *, hipStream_t);
+template void launch_reduce_e<float>(const float *, const float *, const float *, int, int, int, int,
+                                     unsigned long long *, hipStream_t);
+template void launch_slice<double>(const double *, const double *, const double *, const double *, int, char, int,
+                                   int, int, int, int, float *, hipStream_t);
+template void launch_slice<float>(const float *, const float *, const float *, const float *, int, char, int, int,
+                                  int, int, int, float *, hipStream_t);
+
+}  // namespace phpfdtd

@@ -1,0 +1,961 @@
+// phpfdtd_api.hip - C ABI of libphpfdtd_hip (include/phpfdtd.h).
+//
+// A context owns the whole simulation state in HBM and one HIP stream.  One
+// time step is a fixed plan of kernel launches:
+//     h_update -> de_update -> [ade_update per ADE material]
+//              -> [ion_update per lattice material] -> step_end
+// captured once into hipGraphs of 1 and 16 steps and replayed by
+// phpfdtd_run.  The step index lives on the device (advanced by step_end), so
+// replays need no host work.  Sampling, diagnostics and state access run on
+// the same stream outside the graphs.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_precompute.h"
+#include "phpfdtd_internal.h"
+
+namespace phpfdtd {
+template <typename real> void launch_h(const YeeArgs<real> &, hipStream_t);
+template <typename real> void launch_de(const YeeArgs<real> &, int, hipStream_t);
+template <typename real> void launch_ade(const SlotArgs<real> &, hipStream_t);
+template <typename real> void launch_ion(const SlotArgs<real> &, hipStream_t);
+template <typename real> void launch_step_end(const StepEndArgs &, hipStream_t);
+template <typename T> void launch_fill(T *, T, int64_t, hipStream_t);
+template <typename real>
+void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
+                     hipStream_t);
+void launch_reduce_r(const double *, int64_t, unsigned long long *, hipStream_t);
+void launch_argmax_r(const double *, const uint32_t *, int64_t, int, int, int, int, unsigned long long *,
+                     hipStream_t);
+template <typename real>
+void launch_slice(const real *, const real *, const real *, const real *, int, char, int, int, int, int, int,
+                  float *, hipStream_t);
+}  // namespace phpfdtd
+
+using namespace phpfdtd;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return fail(PHPFDTD_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+struct Special {
+    int id = 0;
+    MatPre pre{};
+    std::vector<uint32_t> cells;  // x-fastest field indices, ascending
+    std::vector<double> qezv, qezm, dmm;
+    uint32_t *d_cells = nullptr;
+    void *d_P = nullptr, *d_Po = nullptr;
+    double *d_r = nullptr, *d_v = nullptr;
+    double *d_dmm = nullptr, *d_qezv = nullptr, *d_qezm = nullptr;
+    float *d_dmm32 = nullptr, *d_qezv32 = nullptr, *d_qezm32 = nullptr;
+};
+
+struct Kernel {
+    std::string name;
+    double bytes = 0, flops = 0;
+    std::function<void(hipStream_t)> fn;
+};
+
+// Device bytes of every allocation a context makes; shared by create and
+// phpfdtd_footprint_bytes.
+struct Layout {
+    int nxp = 0;
+    size_t ncellp = 0;
+    size_t field = 0, psi_x = 0, psi_y = 0, psi_z = 0, medium = 0, small = 0, slots = 0, ring = 0;
+    size_t total() const { return 9 * field + 4 * (psi_x + psi_y + psi_z) + medium + small + slots + ring; }
+};
+
+}  // namespace
+
+struct phpfdtd_ctx {
+    phpfdtd_grid g{};
+    int prec = 0;
+    size_t rs = 8;
+    int nx = 0, ny = 0, nz = 0, nxp = 0, L = 0;
+    size_t plane = 0, ncellp = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    void *F[9] = {};
+    void *psi[12] = {};  // Hxy Hxz Dxy Dxz | Hyx Hyz Dyx Dyz | Hzx Hzy Dzx Dzy
+    uint8_t *d_medium = nullptr;
+    std::vector<uint8_t> medium;  // host copy, device layout
+    int nmat = 0;
+    std::vector<MatPre> pre;
+    void *d_mats = nullptr;
+    void *d_axis[3][5] = {};
+    int *d_slot[3] = {};
+    std::vector<Special> specials;
+    int special_of[kMaxMat];
+    // sources
+    int nsrc = 0;
+    int src_i[kMaxSrc] = {}, src_j[kMaxSrc] = {}, src_k[kMaxSrc] = {}, src_c[kMaxSrc] = {};
+    void *d_src_val = nullptr;
+    // step counter and monitor ring
+    int64_t *d_step = nullptr;
+    int64_t host_step = 0;
+    double *d_ring = nullptr;
+    int64_t ring_rows = 0, mon_read = 0;
+    bool mon_on = false;
+    int64_t mon_c = -1, mon_slot = 0, mon_nslot = 0;
+    int mon_dof = -1;
+    const double *mon_r = nullptr, *mon_v = nullptr;
+    unsigned long long *d_diag = nullptr;
+    float *d_slice = nullptr;
+    // plan and graphs
+    std::vector<Kernel> plan;
+    hipGraphExec_t g1 = nullptr, g16 = nullptr;
+    bool dirty = true;
+    bool use_graph = true;
+
+    ~phpfdtd_ctx() {
+        if (!stream && allocs.empty()) return;  // failed before touching the device
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (g1) hipGraphExecDestroy(g1);
+        if (g16) hipGraphExecDestroy(g16);
+        for (void *p : allocs) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    int alloc(void **p, size_t n) {
+        *p = nullptr;
+        if (n == 0) n = 8;
+        hipError_t e = hipMalloc(p, n);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            (void)hipGetLastError();
+            return fail(PHPFDTD_ENOMEM_HBM, "hipMalloc(%zu) failed: %s (context holds %zu bytes)", n,
+                        hipGetErrorString(e), bytes);
+        }
+        allocs.push_back(*p);
+        bytes += n;
+        e = hipMemsetAsync(*p, 0, n, stream);
+        if (e != hipSuccess) return fail(PHPFDTD_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+        return 0;
+    }
+
+    template <typename T>
+    int upload(void **p, const std::vector<T> &v) {
+        int rc = alloc(p, v.size() * sizeof(T));
+        if (rc) return rc;
+        if (!v.empty()) HIPCHK(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        return 0;
+    }
+
+    int upload_real(void **p, const std::vector<double> &v) {
+        if (prec == PHPFDTD_FP64) return upload(p, v);
+        std::vector<float> f(v.begin(), v.end());
+        return upload(p, f);
+    }
+
+    size_t fidx(int i, int j, int k) const { return ((size_t)k * ny + j) * nxp + i; }
+};
+
+static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nslot,
+                          const std::vector<int> &natom, const std::vector<int> &ade) {
+    Layout l;
+    const size_t rs = g.precision == PHPFDTD_FP32 ? 4 : 8;
+    l.nxp = (g.nx + 31) / 32 * 32;
+    l.ncellp = (size_t)l.nxp * g.ny * g.nz;
+    l.field = l.ncellp * rs;
+    const size_t L2 = 2 * (size_t)g.pml_thickness;
+    l.psi_x = L2 * g.ny * g.nz * rs;
+    l.psi_y = L2 * l.nxp * g.nz * rs;
+    l.psi_z = L2 * l.nxp * g.ny * rs;
+    l.medium = l.ncellp;
+    l.small = kMaxMat * sizeof(MatCoef<double>) + 3 * ((size_t)g.nx + g.ny + g.nz) * (5 * rs + 4) +
+              (size_t)g.tsteps * kMaxSrc * rs + 1024 + 64 * 1024;
+    for (size_t m = 0; m < nslot.size(); ++m) {
+        const size_t n = nslot[m];
+        if (!n) continue;
+        l.slots += n * 4;
+        if (ade[m]) l.slots += 2 * 3 * n * rs;
+        if (natom[m]) l.slots += 2 * 3 * (size_t)natom[m] * n * 8 + 9 * natom[m] * natom[m] * 12 + 18 * natom[m] * 12;
+    }
+    l.ring = (size_t)g.tsteps * 12 * 8;
+    return l;
+}
+
+static int check_inputs(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium) {
+    if (!g || !mats || !medium) return fail(PHPFDTD_EINVAL, "null argument");
+    std::string err;
+    if (validate_grid(*g, &err)) return fail(PHPFDTD_EINVAL, "%s", err.c_str());
+    if (nmat < 1 || nmat > kMaxMat) return fail(PHPFDTD_EINVAL, "nmat must be in [1, %d]", kMaxMat);
+    return 0;
+}
+
+// medium (file order) -> per-material cell counts; validates ids (the
+// reference never checks them, fdtd.c:3408).
+static int count_medium(const phpfdtd_grid &g, const int32_t *medium, int nmat, std::vector<int64_t> *cnt) {
+    cnt->assign(nmat, 0);
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    for (size_t c = 0; c < n; ++c) {
+        const int32_t id = medium[c];
+        if (id < 0 || id >= nmat)
+            return fail(PHPFDTD_EFORMAT, "medium id %d at cell %zu is outside the material table [0, %d)", id, c,
+                        nmat);
+        (*cnt)[id]++;
+    }
+    return 0;
+}
+
+template <typename real>
+static void build_plan_t(phpfdtd_ctx *x) {
+    x->plan.clear();
+    const size_t w = sizeof(real);
+    YeeArgs<real> a{};
+    a.Ex = (real *)x->F[0]; a.Ey = (real *)x->F[1]; a.Ez = (real *)x->F[2];
+    a.Dx = (real *)x->F[3]; a.Dy = (real *)x->F[4]; a.Dz = (real *)x->F[5];
+    a.Hx = (real *)x->F[6]; a.Hy = (real *)x->F[7]; a.Hz = (real *)x->F[8];
+    real **ps[12] = {&a.pHxy, &a.pHxz, &a.pDxy, &a.pDxz, &a.pHyx, &a.pHyz,
+                     &a.pDyx, &a.pDyz, &a.pHzx, &a.pHzy, &a.pDzx, &a.pDzy};
+    for (int q = 0; q < 12; ++q) *ps[q] = (real *)x->psi[q];
+    a.medium = x->d_medium;
+    a.mats = (const MatCoef<real> *)x->d_mats;
+    Axis<real> *axes[3] = {&a.ax, &a.ay, &a.az};
+    for (int d = 0; d < 3; ++d) {
+        axes[d]->b = (const real *)x->d_axis[d][0];
+        axes[d]->kh = (const real *)x->d_axis[d][1];
+        axes[d]->kd = (const real *)x->d_axis[d][2];
+        axes[d]->ch = (const real *)x->d_axis[d][3];
+        axes[d]->cd = (const real *)x->d_axis[d][4];
+        axes[d]->slot = x->d_slot[d];
+    }
+    a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp; a.L = x->L;
+    a.nbx = (x->nx + 63) / 64;
+    a.nby = (x->ny + kBlockY - 1) / kBlockY;
+    const int bxy = a.nbx * a.nby;
+    const int target = 2048;
+    int nbz = bxy >= target ? 1 : (target + bxy - 1) / bxy;
+    int kc = (x->nz + nbz - 1) / nbz;
+    if (kc < 4) kc = std::min(4, x->nz);
+    nbz = (x->nz + kc - 1) / kc;
+    a.kchunk = kc;
+    a.nbz = nbz;
+    a.nblocks = bxy * nbz;
+    a.nblocks_pad = (a.nblocks + 7) / 8 * 8;
+    a.nsrc = x->nsrc;
+    for (int s = 0; s < x->nsrc; ++s) {
+        a.src_i[s] = x->src_i[s]; a.src_j[s] = x->src_j[s]; a.src_k[s] = x->src_k[s]; a.src_c[s] = x->src_c[s];
+    }
+    a.src_val = (const real *)x->d_src_val;
+    a.step = x->d_step;
+
+    // algorithmic bytes (compulsory traffic of each kernel's own streams)
+    const double nx = x->nx, ny = x->ny, nz = x->nz, L = x->L;
+    const double cells = nx * ny * nz;
+    const double hcells = (nx - 1) * (ny - 1) * nz;
+    const double icells = (nx - 2) * (ny - 2) * nz;
+    auto pml_inc = [&](double ex, double ey, double ez) {  // (cell, dir) incidences in the region
+        const double lx = std::min(2 * L, ex), ly = std::min(2 * L, ey), lz = std::min(2 * L, nz);
+        return lx * ey * ez + ly * ex * ez + lz * ex * ey;
+    };
+    double special_cells = 0, lossy_int = 0;
+    for (const auto &sp : x->specials) special_cells += (double)sp.cells.size();
+    {
+        // interior cells of lossy materials
+        int64_t n = 0;
+        for (int k = 0; k < x->nz; ++k)
+            for (int j = 1; j < x->ny - 1; ++j)
+                for (int i = 1; i < x->nx - 1; ++i)
+                    if (x->pre[x->medium[x->fidx(i, j, k)]].loss) ++n;
+        lossy_int = (double)n;
+    }
+    Kernel kh;
+    kh.name = "h_update";
+    kh.bytes = hcells * 9 * w + pml_inc(nx - 1, ny - 1, nz) * 4 * w;
+    kh.flops = hcells * 18;
+    kh.fn = [a](hipStream_t s) { launch_h<real>(a, s); };
+    x->plan.push_back(kh);
+    Kernel kd;
+    kd.name = "de_update";
+    kd.bytes = cells * (6 * w + 1) + (cells - special_cells) * 3 * w + icells * 3 * w + lossy_int * 3 * w +
+               pml_inc(nx - 2, ny - 2, nz) * 4 * w;
+    kd.flops = icells * 18 + cells * 15;
+    const int nmat = x->nmat;
+    kd.fn = [a, nmat](hipStream_t s) { launch_de<real>(a, nmat, s); };
+    x->plan.push_back(kd);
+
+    for (auto &sp : x->specials) {
+        SlotArgs<real> sa{};
+        sa.Ex = a.Ex; sa.Ey = a.Ey; sa.Ez = a.Ez;
+        sa.Dx = a.Dx; sa.Dy = a.Dy; sa.Dz = a.Dz;
+        sa.cells = sp.d_cells;
+        sa.nslot = (int64_t)sp.cells.size();
+        for (int e = 0; e < 9; ++e) {
+            sa.R[e] = (real)sp.pre.R[e];
+            sa.A[e] = (real)sp.pre.A[e];
+            sa.Gpi[e] = (real)sp.pre.Gpi[e];
+            sa.Gm[e] = (real)sp.pre.Gm[e];
+            sa.W[e] = (real)sp.pre.W[e];
+        }
+        sa.ade = sp.pre.ade ? 1 : 0;
+        sa.P = (real *)sp.d_P;
+        sa.Po = (real *)sp.d_Po;
+        sa.natom = sp.pre.natom;
+        sa.r = sp.d_r; sa.v = sp.d_v;
+        sa.dmm = sp.d_dmm; sa.qezv = sp.d_qezv; sa.qezm = sp.d_qezm;
+        sa.dmm32 = sp.d_dmm32; sa.qezv32 = sp.d_qezv32; sa.qezm32 = sp.d_qezm32;
+        sa.dt = x->g.dt;
+        sa.half_dt2 = 0.5 * x->g.dt * x->g.dt;
+        const double n = (double)sp.cells.size();
+        Kernel k;
+        char nm[48];
+        if (sp.pre.natom > 0) {
+            const double N3 = 3.0 * sp.pre.natom;
+            snprintf(nm, sizeof(nm), "ion_update[m%d,N%d]", sp.id, sp.pre.natom);
+            k.bytes = n * (6 * w + 4 + 4 * N3 * 8 + (sp.pre.ade ? 12 * w : 0));
+            k.flops = n * (2 * N3 * N3 + 12 * N3 + 40);
+            k.fn = [sa](hipStream_t s) { launch_ion<real>(sa, s); };
+        } else {
+            snprintf(nm, sizeof(nm), "ade_update[m%d]", sp.id);
+            k.bytes = n * (21 * w + 4);
+            k.flops = n * 80;
+            k.fn = [sa](hipStream_t s) { launch_ade<real>(sa, s); };
+        }
+        k.name = nm;
+        x->plan.push_back(k);
+    }
+    StepEndArgs se{};
+    se.step = x->d_step;
+    se.ring = x->d_ring;
+    se.ring_rows = x->ring_rows;
+    se.mon_on = x->mon_on ? 1 : 0;
+    se.mon_c = x->mon_c;
+    for (int q = 0; q < 9; ++q) se.F[q] = x->F[q];
+    se.mon_r = x->mon_r;
+    se.mon_v = x->mon_v;
+    se.mon_slot = x->mon_slot;
+    se.mon_nslot = x->mon_nslot;
+    se.mon_dof = x->mon_dof;
+    se.dt = x->g.dt;
+    Kernel ke;
+    ke.name = "step_end";
+    ke.fn = [se](hipStream_t s) { launch_step_end<real>(se, s); };
+    x->plan.push_back(ke);
+}
+
+static int rebuild(phpfdtd_ctx *x) {
+    if (x->g1) { hipGraphExecDestroy(x->g1); x->g1 = nullptr; }
+    if (x->g16) { hipGraphExecDestroy(x->g16); x->g16 = nullptr; }
+    if (x->prec == PHPFDTD_FP64) build_plan_t<double>(x);
+    else build_plan_t<float>(x);
+    x->dirty = false;
+    if (!x->use_graph) return 0;
+    for (int which = 0; which < 2; ++which) {
+        const int n = which == 0 ? 1 : 16;
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+        for (int t = 0; t < n; ++t)
+            for (auto &k : x->plan) k.fn(x->stream);
+        HIPCHK(hipStreamEndCapture(x->stream, &graph));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(PHPFDTD_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        (which == 0 ? x->g1 : x->g16) = ex;
+    }
+    return 0;
+}
+
+template <typename real>
+static std::vector<real> cast_vec(const std::vector<double> &v) {
+    return std::vector<real>(v.begin(), v.end());
+}
+
+template <typename real>
+static int upload_mats(phpfdtd_ctx *x) {
+    std::vector<MatCoef<real>> mc(x->nmat);
+    for (int m = 0; m < x->nmat; ++m) {
+        memset(&mc[m], 0, sizeof(mc[m]));
+        for (int e = 0; e < 9; ++e) {
+            mc[m].M[e] = (real)x->pre[m].M[e];
+            mc[m].Lk[e] = (real)x->pre[m].Lk[e];
+        }
+        mc[m].flags = (x->pre[m].loss ? MF_LOSS : 0) | (x->pre[m].special ? MF_SPECIAL : 0);
+    }
+    return x->upload(&x->d_mats, mc);
+}
+
+extern "C" {
+
+int phpfdtd_abi_version(void) { return PHPFDTD_ABI_VERSION; }
+
+const char *phpfdtd_last_error(void) { return g_err.c_str(); }
+
+int phpfdtd_footprint_bytes(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
+                            size_t *bytes) {
+    int rc = check_inputs(g, mats, nmat, medium);
+    if (rc) return rc;
+    std::vector<int64_t> cnt;
+    if ((rc = count_medium(*g, medium, nmat, &cnt))) return rc;
+    std::vector<int> natom(nmat), ade(nmat);
+    for (int m = 0; m < nmat; ++m) {
+        natom[m] = mats[m].natom;
+        ade[m] = mats[m].use_ade;
+        if (!natom[m] && !ade[m]) cnt[m] = 0;
+    }
+    *bytes = make_layout(*g, cnt, natom, ade).total();
+    return 0;
+}
+
+int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
+                   phpfdtd_ctx **out) {
+    if (!out) return fail(PHPFDTD_EINVAL, "null out");
+    *out = nullptr;
+    int rc = check_inputs(g, mats, nmat, medium);
+    if (rc) return rc;
+    std::vector<int64_t> cnt;
+    if ((rc = count_medium(*g, medium, nmat, &cnt))) return rc;
+
+    phpfdtd_ctx *x = new phpfdtd_ctx();
+    std::unique_ptr<phpfdtd_ctx> guard(x);
+    x->g = *g;
+    x->prec = g->precision;
+    x->rs = x->prec == PHPFDTD_FP32 ? 4 : 8;
+    x->nx = g->nx; x->ny = g->ny; x->nz = g->nz; x->L = g->pml_thickness;
+    x->nmat = nmat;
+    const char *ng = getenv("PHPFDTD_NO_GRAPH");
+    x->use_graph = !(ng && ng[0] == '1');
+    x->pre.resize(nmat);
+    for (int m = 0; m < nmat; ++m) {
+        std::string err;
+        if ((rc = material_coefs(*g, mats[m], &x->pre[m], &err)))
+            return fail(rc, "material %d: %s", m, err.c_str());
+    }
+    HIPCHK(hipGetDevice(&x->device));
+    HIPCHK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+
+    std::vector<int> natom(nmat), ade(nmat);
+    for (int m = 0; m < nmat; ++m) {
+        natom[m] = mats[m].natom;
+        ade[m] = mats[m].use_ade;
+    }
+    std::vector<int64_t> scnt(cnt);
+    for (int m = 0; m < nmat; ++m)
+        if (!x->pre[m].special) scnt[m] = 0;
+    const Layout lay = make_layout(*g, scnt, natom, ade);
+    x->nxp = lay.nxp;
+    x->ncellp = lay.ncellp;
+    x->plane = (size_t)x->nxp * x->ny;
+
+    for (int q = 0; q < 9; ++q)
+        if ((rc = x->alloc(&x->F[q], lay.field))) return rc;
+    if (x->L > 0) {
+        for (int q = 0; q < 4; ++q)
+            if ((rc = x->alloc(&x->psi[q], lay.psi_x))) return rc;
+        for (int q = 4; q < 8; ++q)
+            if ((rc = x->alloc(&x->psi[q], lay.psi_y))) return rc;
+        for (int q = 8; q < 12; ++q)
+            if ((rc = x->alloc(&x->psi[q], lay.psi_z))) return rc;
+    }
+    // medium: file order [nx][ny][nz] -> x fastest; padding cells get id 0
+    x->medium.assign(x->ncellp, 0);
+    for (int i = 0; i < x->nx; ++i)
+        for (int j = 0; j < x->ny; ++j) {
+            const int32_t *col = medium + ((size_t)i * x->ny + j) * x->nz;
+            for (int k = 0; k < x->nz; ++k) x->medium[x->fidx(i, j, k)] = (uint8_t)col[k];
+        }
+    if ((rc = x->upload((void **)&x->d_medium, x->medium))) return rc;
+    if (x->prec == PHPFDTD_FP64) rc = upload_mats<double>(x);
+    else rc = upload_mats<float>(x);
+    if (rc) return rc;
+    const int ns[3] = {x->nx, x->ny, x->nz};
+    const double ds[3] = {g->dx, g->dy, g->dz};
+    for (int d = 0; d < 3; ++d) {
+        AxisHost ax = axis_tables(ns[d], ds[d], *g);
+        const std::vector<double> *tabs[5] = {&ax.b, &ax.kh, &ax.kd, &ax.ch, &ax.cd};
+        for (int t = 0; t < 5; ++t)
+            if ((rc = x->upload_real(&x->d_axis[d][t], *tabs[t]))) return rc;
+        if ((rc = x->upload((void **)&x->d_slot[d], ax.slot))) return rc;
+    }
+    // slot lists of ADE / lattice materials, x-fastest order
+    for (int m = 0; m < kMaxMat; ++m) x->special_of[m] = -1;
+    for (int m = 0; m < nmat; ++m) {
+        if (!x->pre[m].special || cnt[m] == 0) continue;
+        Special sp;
+        sp.id = m;
+        sp.pre = x->pre[m];
+        sp.cells.reserve(cnt[m]);
+        const int n = mats[m].natom;
+        if (n > 0) {
+            sp.qezv.assign(mats[m].qezv, mats[m].qezv + 9 * n);
+            sp.qezm.assign(mats[m].qezm, mats[m].qezm + 9 * n);
+            sp.dmm.assign(mats[m].dmm, mats[m].dmm + 9 * n * n);
+        }
+        x->special_of[m] = (int)x->specials.size();
+        x->specials.push_back(std::move(sp));
+    }
+    for (size_t c = 0; c < x->ncellp; ++c) {
+        if ((int)(c % x->nxp) >= x->nx) continue;
+        const int so = x->special_of[x->medium[c]];
+        if (so >= 0) x->specials[so].cells.push_back((uint32_t)c);
+    }
+    for (auto &sp : x->specials) {
+        const size_t n = sp.cells.size();
+        if ((rc = x->upload((void **)&sp.d_cells, sp.cells))) return rc;
+        if (sp.pre.ade) {
+            if ((rc = x->alloc(&sp.d_P, 3 * n * x->rs))) return rc;
+            if ((rc = x->alloc(&sp.d_Po, 3 * n * x->rs))) return rc;
+        }
+        if (sp.pre.natom > 0) {
+            const size_t n3 = 3 * (size_t)sp.pre.natom;
+            if ((rc = x->alloc((void **)&sp.d_r, n3 * n * 8))) return rc;
+            if ((rc = x->alloc((void **)&sp.d_v, n3 * n * 8))) return rc;
+            if ((rc = x->upload((void **)&sp.d_dmm, sp.dmm))) return rc;
+            if ((rc = x->upload((void **)&sp.d_qezv, sp.qezv))) return rc;
+            if ((rc = x->upload((void **)&sp.d_qezm, sp.qezm))) return rc;
+            if ((rc = x->upload((void **)&sp.d_dmm32, cast_vec<float>(sp.dmm)))) return rc;
+            if ((rc = x->upload((void **)&sp.d_qezv32, cast_vec<float>(sp.qezv)))) return rc;
+            if ((rc = x->upload((void **)&sp.d_qezm32, cast_vec<float>(sp.qezm)))) return rc;
+        }
+    }
+    if ((rc = x->alloc((void **)&x->d_step, 64))) return rc;
+    x->ring_rows = g->tsteps;
+    if ((rc = x->alloc((void **)&x->d_ring, (size_t)std::max(1, g->tsteps) * 12 * 8))) return rc;
+    if ((rc = x->alloc((void **)&x->d_diag, 64))) return rc;
+    if ((rc = x->alloc(&x->d_src_val, x->rs))) return rc;
+    {
+        const size_t mx = std::max({(size_t)x->nx * x->ny, (size_t)x->ny * x->nz, (size_t)x->nx * x->nz});
+        if ((rc = x->alloc((void **)&x->d_slice, mx * sizeof(float)))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    *out = guard.release();
+    return 0;
+}
+
+void phpfdtd_destroy(phpfdtd_ctx *x) { delete x; }
+
+int64_t phpfdtd_step(const phpfdtd_ctx *x) { return x ? x->host_step : -1; }
+
+int phpfdtd_set_sources(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
+    if (!x || (n > 0 && !src) || n < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    // keep the sources that can ever inject (fdtd.c:3375-3393 skips the rest)
+    std::vector<phpfdtd_source> act;
+    for (int s = 0; s < n; ++s) {
+        const phpfdtd_source &q = src[s];
+        const bool comp_ok = q.component == 'x' || q.component == 'y' || q.component == 'z';
+        if (q.x < 0 || q.x >= x->nx || q.y < 0 || q.y >= x->ny || q.z < 0 || q.z >= x->nz || !comp_ok) continue;
+        act.push_back(q);
+    }
+    if ((int)act.size() > kMaxSrc) return fail(PHPFDTD_EINVAL, "at most %d active sources", kMaxSrc);
+    const int na = (int)act.size();
+    const int T = std::max(1, x->g.tsteps);
+    std::vector<double> val((size_t)T * std::max(1, na), 0.0);
+    for (int t = 0; t < x->g.tsteps; ++t)
+        for (int s = 0; s < na; ++s) val[(size_t)t * na + s] = source_value(act[s], t * x->g.dt) * x->g.dt;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    void *old = x->d_src_val;
+    int rc = x->upload_real(&x->d_src_val, val);
+    if (rc) return rc;
+    auto it = std::find(x->allocs.begin(), x->allocs.end(), old);
+    if (it != x->allocs.end()) {
+        x->allocs.erase(it);
+        hipFree(old);
+    }
+    x->nsrc = na;
+    for (int s = 0; s < na; ++s) {
+        x->src_i[s] = act[s].x;
+        x->src_j[s] = act[s].y;
+        x->src_k[s] = act[s].z;
+        x->src_c[s] = act[s].component - 'x';
+    }
+    x->dirty = true;
+    return 0;
+}
+
+static int find_slot(const phpfdtd_ctx *x, int sp, size_t c, int64_t *slot) {
+    const auto &v = x->specials[sp].cells;
+    auto it = std::lower_bound(v.begin(), v.end(), (uint32_t)c);
+    if (it == v.end() || *it != (uint32_t)c) return -1;
+    *slot = it - v.begin();
+    return 0;
+}
+
+int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int dir, int ref_compat) {
+    if (!x) return fail(PHPFDTD_EINVAL, "null ctx");
+    if (mx < 0 || mx >= x->nx || my < 0 || my >= x->ny || mz < 0 || mz >= x->nz || dir < 0 || dir > 2 || atom < 0)
+        return fail(PHPFDTD_EINVAL, "monitor point (%d,%d,%d) atom %d dir %d out of range", mx, my, mz, atom, dir);
+    const int zf = ref_compat ? mz + 1 : mz;
+    x->mon_on = true;
+    x->mon_c = zf < x->nz ? (int64_t)x->fidx(mx, my, zf) : -1;
+    x->mon_dof = -1;
+    x->mon_r = x->mon_v = nullptr;
+    const int m = x->medium[x->fidx(mx, my, mz)];
+    const int sp = x->special_of[m];
+    if (sp >= 0 && atom < x->pre[m].natom) {
+        int64_t slot = 0;
+        find_slot(x, sp, x->fidx(mx, my, mz), &slot);
+        x->mon_dof = atom * 3 + dir;
+        x->mon_slot = slot;
+        x->mon_nslot = (int64_t)x->specials[sp].cells.size();
+        x->mon_r = x->specials[sp].d_r;
+        x->mon_v = x->specials[sp].d_v;
+    }
+    x->dirty = true;
+    return 0;
+}
+
+int phpfdtd_run(phpfdtd_ctx *x, int nsteps) {
+    if (!x || nsteps < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    if (x->host_step + nsteps > x->g.tsteps)
+        return fail(PHPFDTD_EINVAL, "run past tsteps (%lld + %d > %d)", (long long)x->host_step, nsteps, x->g.tsteps);
+    HIPCHK(hipSetDevice(x->device));
+    if (x->dirty) {
+        int rc = rebuild(x);
+        if (rc) return rc;
+    }
+    int n = nsteps;
+    if (x->use_graph) {
+        while (n >= 16) {
+            HIPCHK(hipGraphLaunch(x->g16, x->stream));
+            n -= 16;
+        }
+        while (n-- > 0) HIPCHK(hipGraphLaunch(x->g1, x->stream));
+    } else {
+        while (n-- > 0)
+            for (auto &k : x->plan) k.fn(x->stream);
+        HIPCHK(hipGetLastError());
+    }
+    x->host_step += nsteps;
+    return 0;
+}
+
+int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
+    if (!x) return fail(PHPFDTD_EINVAL, "null ctx");
+    HIPCHK(hipSetDevice(x->device));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (!d) return 0;
+    unsigned long long init[4] = {0, 0, 0, ~0ull};
+    HIPCHK(hipMemcpyAsync(x->d_diag, init, sizeof(init), hipMemcpyHostToDevice, x->stream));
+    if (x->prec == PHPFDTD_FP64)
+        launch_reduce_e<double>((double *)x->F[0], (double *)x->F[1], (double *)x->F[2], x->nx, x->ny, x->nz, x->nxp,
+                                x->d_diag, x->stream);
+    else
+        launch_reduce_e<float>((float *)x->F[0], (float *)x->F[1], (float *)x->F[2], x->nx, x->ny, x->nz, x->nxp,
+                               x->d_diag, x->stream);
+    for (auto &sp : x->specials)
+        if (sp.pre.natom > 0) launch_reduce_r(sp.d_r, (int64_t)sp.cells.size() * 3 * sp.pre.natom, x->d_diag, x->stream);
+    for (auto &sp : x->specials)
+        if (sp.pre.natom > 0)
+            launch_argmax_r(sp.d_r, sp.d_cells, (int64_t)sp.cells.size(), 3 * sp.pre.natom, x->ny, x->nz, x->nxp,
+                            x->d_diag, x->stream);
+    unsigned long long res[4];
+    HIPCHK(hipMemcpyAsync(res, x->d_diag, sizeof(res), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    memset(d, 0, sizeof(*d));
+    d->step = x->host_step;
+    memcpy(&d->max_e, &res[0], 8);
+    memcpy(&d->max_ex, &res[1], 8);
+    memcpy(&d->max_r, &res[2], 8);
+    d->r_i = d->r_j = d->r_k = d->r_atom = d->r_dir = -1;
+    if (res[3] != ~0ull && d->max_r > 0) {
+        const unsigned long long key = res[3];
+        const int64_t dof = key % 96, cell = key / 96;
+        d->r_k = (int)(cell % x->nz);
+        d->r_j = (int)((cell / x->nz) % x->ny);
+        d->r_i = (int)(cell / ((int64_t)x->nz * x->ny));
+        d->r_atom = (int)(dof / 3);
+        d->r_dir = (int)(dof % 3);
+    }
+    d->diverged = d->max_r > 1e-10;
+    if (d->diverged)
+        return fail(PHPFDTD_EDIVERGED, "ERROR: r field too large at step %lld: max r %e at i=%d, j=%d, k=%d atom %d dir %d",
+                    (long long)x->host_step - 1, d->max_r, d->r_i, d->r_j, d->r_k, d->r_atom, d->r_dir);
+    return 0;
+}
+
+int phpfdtd_read_slice(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, float *out) {
+    if (!x || !out || comp < 0 || comp > 9) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    size_t n;
+    int p = pos;
+    if (axis == 'z') {
+        if (pos < 0 || pos >= x->nz) return fail(PHPFDTD_EINVAL, "Error: No process contains slice z=%d", pos);
+        n = (size_t)x->nx * x->ny;
+        if (ref_compat) p = pos + 1;
+        if (p >= x->nz) {  // the reference reads its zero upper halo plane
+            memset(out, 0, n * sizeof(float));
+            return 0;
+        }
+    } else if (axis == 'x') {
+        if (pos < 0 || pos >= x->nx)
+            return fail(PHPFDTD_EINVAL, "Error: Invalid x slice position %d (range: 0-%d)", pos, x->nx - 1);
+        n = (size_t)x->ny * x->nz;
+    } else if (axis == 'y') {
+        if (pos < 0 || pos >= x->ny)
+            return fail(PHPFDTD_EINVAL, "Error: Invalid y slice position %d (range: 0-%d)", pos, x->ny - 1);
+        n = (size_t)x->nx * x->nz;
+    } else {
+        return fail(PHPFDTD_EINVAL, "Error: Invalid slice axis '%c'. Must be 'x', 'y', or 'z'.", axis);
+    }
+    float *buf = x->d_slice;
+    const int emag = comp == PHPFDTD_EMAG;
+    const int fc = emag ? 0 : comp;
+    if (x->prec == PHPFDTD_FP64)
+        launch_slice<double>((double *)x->F[fc], (double *)x->F[0], (double *)x->F[1], (double *)x->F[2], emag, axis,
+                             p, x->nx, x->ny, x->nz, x->nxp, (float *)buf, x->stream);
+    else
+        launch_slice<float>((float *)x->F[fc], (float *)x->F[0], (float *)x->F[1], (float *)x->F[2], emag, axis, p,
+                            x->nx, x->ny, x->nz, x->nxp, (float *)buf, x->stream);
+    HIPCHK(hipMemcpyAsync(out, buf, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int phpfdtd_read_monitor(phpfdtd_ctx *x, double *rows, int max_rows, int *n_rows) {
+    if (!x || !n_rows || max_rows < 0 || (max_rows > 0 && !rows)) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    *n_rows = 0;
+    if (!x->mon_on) return 0;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    const int64_t avail = std::min(x->host_step, x->ring_rows) - x->mon_read;
+    const int64_t n = std::min<int64_t>(avail, max_rows);
+    if (n <= 0) return 0;
+    HIPCHK(hipMemcpy(rows, x->d_ring + x->mon_read * 12, n * 12 * sizeof(double), hipMemcpyDeviceToHost));
+    x->mon_read += n;
+    *n_rows = (int)n;
+    return 0;
+}
+
+int phpfdtd_read_point(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
+    if (!x || !f9 || px < 0 || px >= x->nx || py < 0 || py >= x->ny || pz < 0 || pz > x->nz)
+        return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int q = 0; q < 9; ++q) f9[q] = 0.0;
+    if (pz == x->nz) return 0;
+    const size_t c = x->fidx(px, py, pz);
+    for (int q = 0; q < 9; ++q) {
+        if (x->prec == PHPFDTD_FP64) {
+            HIPCHK(hipMemcpy(&f9[q], (double *)x->F[q] + c, 8, hipMemcpyDeviceToHost));
+        } else {
+            float v;
+            HIPCHK(hipMemcpy(&v, (float *)x->F[q] + c, 4, hipMemcpyDeviceToHost));
+            f9[q] = v;
+        }
+    }
+    return 0;
+}
+
+int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double *v) {
+    if (!x || px < 0 || px >= x->nx || py < 0 || py >= x->ny || pz < 0 || pz >= x->nz)
+        return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    const size_t c = x->fidx(px, py, pz);
+    const int m = x->medium[c];
+    const int sp = x->special_of[m];
+    if (sp < 0 || x->pre[m].natom == 0) return 0;
+    int64_t slot = 0;
+    if (find_slot(x, sp, c, &slot)) return fail(PHPFDTD_EINVAL, "cell has no slot");
+    const Special &s = x->specials[sp];
+    const size_t n = s.cells.size();
+    const int n3 = 3 * x->pre[m].natom;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (r) HIPCHK(hipMemcpy2D(r, 8, s.d_r + slot, n * 8, 8, n3, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(hipMemcpy2D(v, 8, s.d_v + slot, n * 8, 8, n3, hipMemcpyDeviceToHost));
+    return x->pre[m].natom;
+}
+
+int phpfdtd_read_field(phpfdtd_ctx *x, int comp, double *out) {
+    if (!x || !out || comp < 0 || comp > 8) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    std::vector<unsigned char> h(x->ncellp * x->rs);
+    HIPCHK(hipMemcpy(h.data(), x->F[comp], h.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < x->nx; ++i)
+        for (int j = 0; j < x->ny; ++j)
+            for (int k = 0; k < x->nz; ++k) {
+                const size_t c = x->fidx(i, j, k);
+                out[((size_t)i * x->ny + j) * x->nz + k] =
+                    x->prec == PHPFDTD_FP64 ? ((double *)h.data())[c] : (double)((float *)h.data())[c];
+            }
+    return 0;
+}
+
+int phpfdtd_write_field(phpfdtd_ctx *x, int comp, const double *in) {
+    if (!x || !in || comp < 0 || comp > 8) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    std::vector<unsigned char> h(x->ncellp * x->rs, 0);
+    for (int i = 0; i < x->nx; ++i)
+        for (int j = 0; j < x->ny; ++j)
+            for (int k = 0; k < x->nz; ++k) {
+                const size_t c = x->fidx(i, j, k);
+                const double v = in[((size_t)i * x->ny + j) * x->nz + k];
+                if (x->prec == PHPFDTD_FP64) ((double *)h.data())[c] = v;
+                else ((float *)h.data())[c] = (float)v;
+            }
+    HIPCHK(hipMemcpy(x->F[comp], h.data(), h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int phpfdtd_count_cells(phpfdtd_ctx *x, int mat, int64_t *n) {
+    if (!x || !n || mat < 0 || mat >= x->nmat) return fail(PHPFDTD_EINVAL, "bad arguments");
+    const int sp = x->special_of[mat];
+    *n = sp >= 0 ? (int64_t)x->specials[sp].cells.size() : 0;
+    return 0;
+}
+
+// order of a material's slots in file order (i slowest, k fastest)
+static std::vector<int64_t> file_order(const phpfdtd_ctx *x, const Special &s) {
+    std::vector<int64_t> idx(s.cells.size());
+    std::vector<uint64_t> key(s.cells.size());
+    for (size_t q = 0; q < s.cells.size(); ++q) {
+        const uint64_t c = s.cells[q];
+        const uint64_t i = c % x->nxp, j = (c / x->nxp) % x->ny, k = c / ((uint64_t)x->nxp * x->ny);
+        key[q] = (i * x->ny + j) * x->nz + k;
+        idx[q] = (int64_t)q;
+    }
+    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    return idx;
+}
+
+static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
+    if (!x || !buf || mat < 0 || mat >= x->nmat || kind < 0 || kind > 3) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    const int sp = x->special_of[mat];
+    if (sp < 0) return 0;
+    Special &s = x->specials[sp];
+    const size_t n = s.cells.size();
+    int width;
+    void *dev;
+    size_t es;
+    if (kind < 2) {
+        if (!s.pre.natom) return fail(PHPFDTD_EINVAL, "material %d has no lattice", mat);
+        width = 3 * s.pre.natom;
+        dev = kind == 0 ? (void *)s.d_r : (void *)s.d_v;
+        es = 8;
+    } else {
+        if (!s.pre.ade) return fail(PHPFDTD_EINVAL, "material %d has no ADE state", mat);
+        width = 3;
+        dev = kind == 2 ? s.d_P : s.d_Po;
+        es = x->rs;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    std::vector<unsigned char> h(n * width * es);
+    const std::vector<int64_t> ord = file_order(x, s);
+    if (!write) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) {
+        const int64_t slot = ord[q];
+        for (int d = 0; d < width; ++d) {
+            const size_t hi = (size_t)d * n + slot;
+            double &u = buf[q * width + d];
+            if (es == 8) {
+                if (write) ((double *)h.data())[hi] = u;
+                else u = ((double *)h.data())[hi];
+            } else {
+                if (write) ((float *)h.data())[hi] = (float)u;
+                else u = ((float *)h.data())[hi];
+            }
+        }
+    }
+    if (write) HIPCHK(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int phpfdtd_read_state(phpfdtd_ctx *x, int mat, int kind, double *out) { return state_io(x, mat, kind, out, 0); }
+
+int phpfdtd_write_state(phpfdtd_ctx *x, int mat, int kind, const double *in) {
+    return state_io(x, mat, kind, const_cast<double *>(in), 1);
+}
+
+int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out, int *n_out) {
+    if (!x || nsteps <= 0 || !n_out) return fail(PHPFDTD_EINVAL, "bad arguments");
+    if (x->host_step + nsteps > x->g.tsteps) return fail(PHPFDTD_EINVAL, "profile past tsteps");
+    HIPCHK(hipSetDevice(x->device));
+    if (x->dirty) {
+        int rc = rebuild(x);
+        if (rc) return rc;
+    }
+    const size_t nk = x->plan.size();
+    std::vector<hipEvent_t> ev((size_t)nsteps * nk * 2);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (int t = 0; t < nsteps; ++t)
+        for (size_t k = 0; k < nk; ++k) {
+            HIPCHK(hipEventRecord(ev[((size_t)t * nk + k) * 2], x->stream));
+            x->plan[k].fn(x->stream);
+            HIPCHK(hipEventRecord(ev[((size_t)t * nk + k) * 2 + 1], x->stream));
+        }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->host_step += nsteps;
+    *n_out = 0;
+    for (size_t k = 0; k < nk && (int)k < max_out; ++k) {
+        double tot = 0;
+        for (int t = 0; t < nsteps; ++t) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[((size_t)t * nk + k) * 2], ev[((size_t)t * nk + k) * 2 + 1]));
+            tot += ms;
+        }
+        phpfdtd_ktime &o = out[k];
+        memset(&o, 0, sizeof(o));
+        snprintf(o.name, sizeof(o.name), "%s", x->plan[k].name.c_str());
+        o.avg_ms = tot / nsteps;
+        o.bytes = x->plan[k].bytes;
+        o.flops = x->plan[k].flops;
+        o.launches = 1;
+        *n_out = (int)k + 1;
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    return 0;
+}
+
+int phpfdtd_time_steps(phpfdtd_ctx *x, int nsteps, double *ms) {
+    if (!x || !ms || nsteps <= 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(x->device));
+    if (x->dirty) {
+        int rc = rebuild(x);
+        if (rc) return rc;
+    }
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipEventRecord(a, x->stream));
+    int rc = phpfdtd_run(x, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b, x->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = t / nsteps;
+    return 0;
+}
+
+int phpfdtd_struct_sizes(int *sizes, int n) {
+    const int s[5] = {(int)sizeof(phpfdtd_grid), (int)sizeof(phpfdtd_material), (int)sizeof(phpfdtd_source),
+                      (int)sizeof(phpfdtd_diag), (int)sizeof(phpfdtd_ktime)};
+    for (int q = 0; q < n && q < 5; ++q) sizes[q] = s[q];
+    return 5;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// phpfdtd_internal.h - device data layout and kernel parameter blocks of
+// libphpfdtd_hip.  Not part of the public ABI (include/phpfdtd.h).
+//
+// Device layout (HBM), chosen for gfx950 rather than copied from fdtd.c:
+//   * fields Ex..Hz: one array per component, x fastest, row pitch nxp
+//     (nx rounded up to 32 elements), cell (i,j,k) at (k*ny + j)*nxp + i.  A
+//     64-lane wavefront reads 64 consecutive x cells of one row, so every
+//     field stream is a fully coalesced 256 B (FP32) / 512 B (FP64) access.
+//   * material ids: uint8 per cell, same layout.
+//   * CPML psi: compact slabs, not full volumes:
+//       x slabs [nz][ny][2L], y slabs [nz][2L][nxp], z slabs [2L][ny][nxp].
+//   * ADE and lattice state: per-material slot arrays (SoA), slot s of
+//     material m is the s-th cell of m in x-fastest order; r[dof][slot].
+// FP64 host precompute (CPML b/c/kappa, eps^-1, ADE coefficients, rotations,
+// source values) is cast to the working type only when uploaded.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/phpfdtd.h"
+
+namespace phpfdtd {
+
+constexpr int kMaxMat = 64;     // material ids are stored as uint8
+constexpr int kMaxSrc = 8;      // sources handled inside the D/E kernel
+constexpr int kMaxAtom = 32;    // lattice atoms per cell
+constexpr int kBlockY = 4;      // rows per Yee work-group (64 x 4 threads)
+constexpr double kEpsInf = 20.0;  // EPS_INF of the reference ADE (fdtd.c:202)
+
+enum MatFlags : int {
+    MF_LOSS = 1,     // eps_im != 0 and no ADE: D -= Lk . E^n (fdtd.c:3349-3353)
+    MF_SPECIAL = 2,  // E is produced by a slot kernel (ADE and/or lattice)
+};
+
+// Per-material coefficients read by the fused D/E kernel.
+template <typename real>
+struct MatCoef {
+    real M[9];   // plain cells: E_lab = M . D_lab, M = R^T (lf (1-lt) Re(eps^-1)/eps0) R
+    real Lk[9];  // D loss: D -= Lk . E^n_lab, Lk = dt f 2 3.14159265 eps0 eps_im
+    int flags;
+    int pad[3];
+};
+
+// One CPML axis: coefficient tables over the axis index (length n).
+template <typename real>
+struct Axis {
+    const real *b;    // exp(-(sigma/(kappa eps0) + alpha/eps0) dt)
+    const real *kh;   // H curl coefficient dt / (mu0 delta kappa)
+    const real *kd;   // D curl coefficient dt / (delta kappa)
+    const real *ch;   // H psi coefficient dt c / (mu0 delta)
+    const real *cd;   // D psi coefficient dt c / delta
+    const int *slot;  // psi slab index 0..2L-1, or -1 outside the CPML
+};
+
+template <typename real>
+struct YeeArgs {
+    real *Ex, *Ey, *Ez, *Dx, *Dy, *Dz, *Hx, *Hy, *Hz;
+    real *pHxy, *pHxz, *pDxy, *pDxz;  // x slabs
+    real *pHyx, *pHyz, *pDyx, *pDyz;  // y slabs
+    real *pHzx, *pHzy, *pDzx, *pDzy;  // z slabs
+    const uint8_t *medium;
+    const MatCoef<real> *mats;
+    Axis<real> ax, ay, az;
+    int nx, ny, nz, nxp, L;
+    int kchunk, nbx, nby, nbz, nblocks, nblocks_pad;
+    // sources: value*dt per step in src_val[step*nsrc + s]
+    int nsrc;
+    int src_i[kMaxSrc], src_j[kMaxSrc], src_k[kMaxSrc], src_c[kMaxSrc];
+    const real *src_val;
+    const int64_t *step;
+};
+
+// Lattice / ADE slot kernels: one material per launch.
+template <typename real>
+struct SlotArgs {
+    real *Ex, *Ey, *Ez;
+    const real *Dx, *Dy, *Dz;
+    const uint32_t *cells;  // slot -> field index
+    int64_t nslot;
+    // crystal-frame coefficients
+    real R[9];    // lab -> crystal rotation
+    real A[9];    // E_cr = A . (D_cr - P)  (lf, 1-lt, Re(eps^-1), 1/eps0 folded)
+    // ADE (fdtd.c:3463-3479), P^{n+1} = Gpi (2P - Gm P^{n-1} + W E^n_cr)
+    int ade;
+    real Gpi[9], Gm[9], W[9];
+    real *P, *Po;  // [3][nslot]
+    // lattice (fdtd.c:3505-3528)
+    int natom;
+    double *r, *v;           // [3N][nslot], FP64 state in both forms
+    const double *qezv;      // [N][3][3]
+    const double *qezm;      // [N][3][3]
+    const double *dmm;       // [3N][3N]
+    const float *dmm32;      // FP32 copy [3N][3N] (precision form)
+    const float *qezv32, *qezm32;
+    double dt, half_dt2;
+};
+
+// Per-step epilogue: monitor row + step counter.
+struct StepEndArgs {
+    int64_t *step;
+    double *ring;      // [tsteps][12]
+    int64_t ring_rows;
+    int mon_on;
+    int64_t mon_c;     // field index sampled (z+1 in reference mode), -1 = zero halo
+    const void *F[9];  // field arrays (real)
+    const double *mon_r, *mon_v;  // lattice state of the monitor cell's material
+    int64_t mon_slot, mon_nslot;
+    int mon_dof;       // atom*3+dir, -1 = no R/V
+    double dt;
+};
+
+}  // namespace phpfdtd

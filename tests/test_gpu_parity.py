@@ -1,0 +1,210 @@
+"""Parity of the HIP path (through the C ABI) with the reference and the oracle.
+
+Bars (stated here, checked below):
+  * FP64 plain form vs the reference's golden runs: monitor traces and
+    full-monitor rows within 1e-10 relative L2, float32 slices within 1 ulp.
+  * FP64 plain form vs the CPU oracle from random states (every term active):
+    every field, lattice and ADE state within 1e-12 relative L2 per component.
+  * FP32 precision form vs the golden FP64 traces: relative L2 <= 1e-4.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import golden_cases
+from oracle import orc
+from php_fdtd_amd import _abi
+from php_fdtd_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+RANDOM_TOL = 1e-12
+FP32_TOL = 1e-4
+
+
+def rel_l2(got, ref):
+    ref = np.asarray(ref, np.float64)
+    got = np.asarray(got, np.float64)
+    nr = np.linalg.norm(ref)
+    if nr == 0:
+        return np.linalg.norm(got)
+    return np.linalg.norm(got - ref) / nr
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, -(a & 0x7FFFFFFF), a)
+    b = np.where(b < 0, -(b & 0x7FFFFFFF), b)
+    return np.abs(a - b)
+
+
+def run_case(c, precision):
+    spec = c["spec"]
+    print("case", spec["name"], "precision", precision, flush=True)
+    sim = Simulation(spec, c["medium"], precision)
+    mon = spec["monitor"]
+    out = spec["output"]
+    sim.set_monitor(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"], ref_compat=True)
+    T = spec["grid"]["tsteps"]
+    slices, full = {}, []
+    t = 0
+    while t < T:
+        if t % out["interval"] == 0:
+            sim.run(1)
+            slices[t] = sim.slice(out["component"], out["axis"], out["pos"])
+            row = list(sim.point(mon["x"], mon["y"], min(mon["z"] + 1, spec["grid"]["nz"])))
+            r, v = sim.ions(mon["x"], mon["y"], mon["z"])
+            for a in range(len(r) // 3):
+                row += list(r[3 * a:3 * a + 3]) + list(v[3 * a:3 * a + 3])
+            full.append(row)
+            t += 1
+        else:
+            n = min(out["interval"] - t % out["interval"], T - t)
+            sim.run(n)
+            t += n
+    rows = sim.monitor()
+    d = sim.sync()
+    sim.close()
+    return rows, slices, np.array(full), d
+
+
+@pytest.fixture(scope="module", params=golden_cases.CASES)
+def golden(request):
+    return golden_cases.load(request.param)
+
+
+def test_golden_fp64(golden):
+    c = golden
+    rows, slices, full, d = run_case(c, _abi.FP64)
+    ref = c["monitor"]
+    assert rows.shape[0] == ref.shape[0]
+    np.testing.assert_array_equal(rows[:, 0], ref[:, 0])
+    assert rel_l2(rows[:, 1:10], ref[:, 1:10]) <= FP64_TOL
+    if ref.shape[1] > 10:
+        assert rel_l2(rows[:, 10:12], ref[:, 10:12]) <= FP64_TOL
+    for t, s in c["slices"].items():
+        assert ulp_diff(slices[t], s).max() <= 1, "slice at step %d" % t
+    assert full.shape == c["full"].shape
+    assert rel_l2(full[:, 1:], c["full"][:, 1:]) <= FP64_TOL
+    assert d.diverged == 0
+
+
+def test_golden_fp32(golden):
+    c = golden
+    rows, slices, full, d = run_case(c, _abi.FP32)
+    ref = c["monitor"]
+    if np.abs(ref[:, 1:10]).max() > 0:
+        err = rel_l2(rows[:, 1:10], ref[:, 1:10])
+        print("fp32 monitor rel L2", err)
+        assert err <= FP32_TOL
+    if ref.shape[1] > 10:
+        assert rel_l2(rows[:, 10:12], ref[:, 10:12]) <= FP32_TOL
+
+
+def randomize(spec, medium, seed, sim, o):
+    rng = np.random.default_rng(seed)
+    shape = medium.shape
+    for comp in range(9):
+        f = rng.standard_normal(shape)
+        if comp >= 3 and comp < 6:
+            f *= 8.854e-12  # D ~ eps0 E
+        elif comp >= 6:
+            f /= 377.0
+        sim.set_field(comp, f)
+        o.set_field(comp, f)
+    for m, mat in enumerate(spec["materials"]):
+        n = sim.count_cells(m)
+        if n == 0:
+            continue
+        if mat["natom"] > 0:
+            for kind, scale in ((0, 1e-13), (1, 1e-3)):
+                a = rng.standard_normal(n * 3 * mat["natom"]) * scale
+                sim.set_state(m, kind, a)
+                o.set_state(m, kind, a)
+        if mat["use_ade"]:
+            for kind in (2, 3):
+                a = rng.standard_normal(n * 3) * 1e-11
+                sim.set_state(m, kind, a)
+                o.set_state(m, kind, a)
+
+
+@pytest.mark.parametrize("name", golden_cases.CASES)
+def test_random_state_vs_oracle(name):
+    c = golden_cases.load(name)
+    spec = c["spec"]
+    print("random-state case", name, flush=True)
+    sim = Simulation(spec, c["medium"], _abi.FP64)
+    o = orc.Oracle(spec, c["medium"])
+    randomize(spec, c["medium"], 1234, sim, o)
+    nstep = 5
+    sim.run(nstep)
+    o.step(nstep)
+    for comp in range(9):
+        e = rel_l2(sim.field(comp), o.field(comp))
+        assert e <= RANDOM_TOL, "component %d rel err %g" % (comp, e)
+    for m, mat in enumerate(spec["materials"]):
+        if sim.count_cells(m) == 0:
+            continue
+        kinds = ([0, 1] if mat["natom"] else []) + ([2, 3] if mat["use_ade"] else [])
+        for kind in kinds:
+            e = rel_l2(sim.state(m, kind), o.state(m, kind))
+            assert e <= RANDOM_TOL, "material %d kind %d rel err %g" % (m, kind, e)
+    d = sim.sync()
+    v, idx = o.diag()
+    assert abs(d.max_e - v[0]) <= 1e-12 * v[0]
+    assert abs(d.max_ex - v[1]) <= 1e-12 * v[1]
+    assert abs(d.max_r - v[2]) <= 1e-12 * max(v[2], 1e-300)
+    if v[2] > 0:
+        assert [d.r_i, d.r_j, d.r_k, d.r_atom, d.r_dir] == idx
+    sim.close()
+
+
+def test_graph_matches_stream_launches(monkeypatch):
+    c = golden_cases.load("hbn")
+    spec = c["spec"]
+    res = []
+    for ng in ("0", "1"):
+        monkeypatch.setenv("PHPFDTD_NO_GRAPH", ng)
+        sim = Simulation(spec, c["medium"], _abi.FP64)
+        sim.run(37)  # 2 x 16-step graph + 5 x 1-step graph
+        res.append([sim.field(q) for q in range(9)])
+        sim.close()
+    for q in range(9):
+        np.testing.assert_array_equal(res[0][q], res[1][q])
+
+
+def test_errors_and_bounds():
+    c = golden_cases.load("vacuum_cpml")
+    spec = c["spec"]
+    sim = Simulation(spec, c["medium"], _abi.FP64)
+    with pytest.raises(_abi.PhpFdtdError) as ei:
+        sim.run(spec["grid"]["tsteps"] + 1)
+    assert ei.value.code == _abi.EINVAL
+    with pytest.raises(_abi.PhpFdtdError):
+        sim.slice("Ex", "q", 0)
+    with pytest.raises(_abi.PhpFdtdError):
+        sim.slice("Ex", "x", spec["grid"]["nx"])
+    sim.run(3)
+    assert sim.t == 3
+    # the z slice of the top plane reads the zero halo, as the reference does
+    top = sim.slice("Ez", "z", spec["grid"]["nz"] - 1)
+    assert np.all(top == 0)
+    sim.close()
+
+
+def test_state_roundtrip():
+    c = golden_cases.load("moo3")
+    spec = c["spec"]
+    sim = Simulation(spec, c["medium"], _abi.FP64)
+    rng = np.random.default_rng(7)
+    n = sim.count_cells(4)
+    a = rng.standard_normal(n * 48)
+    sim.set_state(4, 0, a)
+    np.testing.assert_array_equal(sim.state(4, 0), a)
+    f = rng.standard_normal(sim.shape)
+    sim.set_field(5, f)
+    np.testing.assert_array_equal(sim.field(5), f)
+    sim.close()
