@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-timestep update on one MI355X.
+
+Workload: the 1024x1024x640 configuration (10 nm cells, 20-cell CPML, 48-cell
+16-atom lattice slab, Drude-metal cone tip, Ricker source), synthetic seeded
+materials, all state resident in HBM before timing.  A "step" is one full time
+step (H, D/E, ADE, lattice, monitor epilogue) of the whole domain.
+
+Reported: Gcell-updates/s of the FP32 precision form (value) and of the FP64
+plain form, the HBM roofline of the dominant kernel (HIP events on the
+library's stream, algorithmic bytes), the whole-step byte model, the FP32 vs
+FP64 monitor error over the timed run, and the reference solver on the host
+cores on a bounded sample of the same workload (cpu_baseline).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3]
+"""
+import argparse
+import json
+import math
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s + HBM roofline fraction; solves/s, p50/p99, FP32 vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(spec, med, precision, warmup, steps, prof_steps):
+    from php_fdtd_amd import _abi
+    from php_fdtd_amd.sim import Simulation
+
+    t0 = time.time()
+    sim = Simulation(spec, med, precision)
+    mon = spec["monitor"]
+    sim.set_monitor(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"], ref_compat=True)
+    log("  context ready in %.1f s" % (time.time() - t0))
+    sim.run(warmup)
+    sim.sync(diag=False)
+    wall0 = time.perf_counter()
+    ms = sim.time_steps(steps)  # HIP events on the library stream, synced on both sides
+    wall = time.perf_counter() - wall0
+    prof = sim.profile(prof_steps)
+    rows = sim.monitor()
+    d = sim.sync()
+    sim.close()
+    return dict(ms=ms, wall_ms=wall * 1e3 / steps, prof=prof, rows=rows, max_e=d.max_e, max_r=d.max_r)
+
+
+def cpu_baseline(budget_s=25.0):
+    """Reference fdtd.c (oracle/_ref/fdtd.x, -Ofast -flto -funroll-loops) under
+    mpirun on the host cores, on the same workload scaled down 4x per axis
+    (256x256x160 cells, same slab/cone/CPML layout), for a step count sized to
+    ~budget_s of CPU time.  Falls back to the single-thread C oracle
+    (oracle/liborc.so) when the reference binary is absent."""
+    from php_fdtd_amd import workloads
+
+    ref = os.path.join(ROOT, "oracle", "_ref", "fdtd.x")
+    mpirun = "/opt/conda/bin/mpirun"
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except Exception:
+        ncpu = os.cpu_count() or 1
+    cores = max(1, min(16, ncpu))
+    spec, med = workloads.config3(scale=4)
+    cells = med.size
+    if os.path.exists(ref) and os.path.exists(mpirun):
+        # probe 2 steps to size the sample
+        tmp = tempfile.mkdtemp(prefix="phpfdtd_cpu_")
+        try:
+            def run(nsteps):
+                sp = workloads.with_steps(spec, nsteps)
+                cfg = workloads.write_reference_case(tmp, sp, med)
+                env = dict(os.environ, OMP_NUM_THREADS="1")
+                p = subprocess.run([mpirun, "-np", str(cores), ref, cfg], cwd=tmp, capture_output=True, text=True,
+                                   timeout=600, env=env)
+                out = p.stdout
+                for ln in out.splitlines():
+                    if "Simulation completed in" in ln:
+                        return float(ln.split("in")[1].split()[0])
+                raise RuntimeError("reference run failed rc=%d: %s" % (p.returncode, (out + p.stderr)[-800:]))
+            t2 = run(2)
+            n = int(max(3, min(400, budget_s / max(t2 / 2, 1e-3))))
+            t = run(n)
+            return dict(value=cells * n / t / 1e9, unit="Gcell-updates/s", cores=cores, kind="reference",
+                        sample="reference fdtd.c (-Ofast -flto -funroll-loops) under mpirun -np %d, "
+                               "256x256x160 (config 3 scaled 1/4 per axis), %d steps, loop time %.2f s" % (cores, n, t))
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    from oracle import orc
+
+    spec, med = workloads.config3(scale=8)
+    spec = workloads.with_steps(spec, 1000)
+    o = orc.Oracle(spec, med)
+    t0 = time.time()
+    n = 0
+    while time.time() - t0 < budget_s / 2 and n < 1000:
+        o.step(1)
+        n += 1
+    t = time.time() - t0
+    return dict(value=med.size * n / t / 1e9, unit="Gcell-updates/s", cores=1, kind="port",
+                sample="C oracle (FP64 restatement), 1 thread, 128x128x80 (config 3 scaled 1/8), %d steps" % n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp32,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the domain is resident on a single MI355X and no "
+              "multi-GPU path is built (--gpus %d not supported)" % args.gpus, file=sys.stderr)
+        return 2
+
+    from php_fdtd_amd import _abi, workloads
+
+    prof_steps = 3
+    t0 = time.time()
+    spec, med = workloads.CONFIGS[args.config]()
+    spec = workloads.with_steps(spec, args.warmup + args.steps + prof_steps)
+    log("workload %s: %s cells, built in %.1f s" % (spec["name"], med.size, time.time() - t0))
+    cells = float(med.size)
+    res = {}
+    for form in args.forms.split(","):
+        prec = _abi.FP32 if form == "fp32" else _abi.FP64
+        log("form %s" % form)
+        res[form] = run_form(spec, med, prec, args.warmup, args.steps, prof_steps)
+        log("  %.3f ms/step (events), %.3f ms/step (wall)" % (res[form]["ms"], res[form]["wall_ms"]))
+        for k in res[form]["prof"]:
+            log("    %-24s %9.3f ms  %8.1f GB/s" % (k["name"], k["ms"], k["bytes"] / k["ms"] / 1e6 if k["ms"] else 0))
+
+    head = "fp32" if "fp32" in res else "fp64"
+    r = res[head]
+    word = 4 if head == "fp32" else 8
+    gcell = cells / (r["ms"] * 1e-3) / 1e9
+    dom = max(r["prof"], key=lambda k: k["ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    sb = workloads.step_bytes(spec, med, word)
+    out = {
+        "metric": METRIC,
+        "value": round(gcell, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "%s: %dx%dx%d cells, 16-atom lattice slab + Drude cone, %s form (fields/CPML/ADE "
+                               "%s, lattice r,v f64)" % (spec["name"], spec["grid"]["nx"], spec["grid"]["ny"],
+                                                        spec["grid"]["nz"], head.upper(), head),
+                   "cells": int(cells), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "step_roofline": {"bytes_per_step": sb, "achieved": round(sb / (r["ms"] * 1e-3) / 1e9, 1),
+                          "frac": round(sb / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "model": "C(18w+1) + 8w*CPML incidences + 9w*ADE cells + 96N*lattice cells"},
+        "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r["prof"]},
+    }
+    if "fp64" in res and head != "fp64":
+        r64 = res["fp64"]
+        out["fp64"] = {"value": round(cells / (r64["ms"] * 1e-3) / 1e9, 4), "ms_per_step": round(r64["ms"], 4),
+                       "step_frac": round(workloads.step_bytes(spec, med, 8) / (r64["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r64["prof"]}}
+        out["speedup_fp32_vs_fp64"] = round(r64["ms"] / r["ms"], 3)
+        a, b = res["fp32"]["rows"], r64["rows"]
+        n = min(len(a), len(b))
+        if n:
+            num = np.linalg.norm(a[:n, 1:10] - b[:n, 1:10])
+            den = np.linalg.norm(b[:n, 1:10])
+            out["fp32_monitor_rel_l2_vs_fp64"] = float(num / den) if den else None
+    if not args.no_cpu:
+        try:
+            t0 = time.time()
+            out["cpu_baseline"] = cpu_baseline()
+            log("cpu baseline %.1f s: %s" % (time.time() - t0, out["cpu_baseline"]))
+        except Exception as e:  # reported, not fatal
+            out["cpu_baseline"] = {"value": None, "error": str(e)[:300]}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -649,7 +649,8 @@ void orc_set_field(orc_sim *s, int comp, const double *in) {
 }
 
 /* Ion / ADE state of all cells of material mat, in file order
- * (kind 0 r, 1 v, 2 P, 3 Po).  Returns the number of values. */
+ * (kind 0 r, 1 v, 2 P, 3 Po).  Returns the number of values; with buf NULL it
+ * only counts them. */
 long orc_state(orc_sim *s, int mat, int kind, double *buf, int write) {
     long n = 0;
     size_t ncell = (size_t)s->nx * s->ny * s->nz;
@@ -668,8 +669,10 @@ long orc_state(orc_sim *s, int mat, int kind, double *buf, int write) {
             w = 3;
         }
         for (int q = 0; q < w; q++) {
-            if (write) p[q] = buf[n];
-            else buf[n] = p[q];
+            if (buf) {
+                if (write) p[q] = buf[n];
+                else buf[n] = p[q];
+            }
             n++;
         }
     }

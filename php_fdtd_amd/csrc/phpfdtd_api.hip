@@ -112,6 +112,7 @@ struct phpfdtd_ctx {
     int *d_slot[3] = {};
     std::vector<Special> specials;
     int special_of[kMaxMat];
+    int64_t lossy_interior = 0;  // interior cells of materials with eps_im loss
     // sources
     int nsrc = 0;
     int src_i[kMaxSrc] = {}, src_j[kMaxSrc] = {}, src_k[kMaxSrc] = {}, src_c[kMaxSrc] = {};
@@ -276,17 +277,9 @@ static void build_plan_t(phpfdtd_ctx *x) {
         const double lx = std::min(2 * L, ex), ly = std::min(2 * L, ey), lz = std::min(2 * L, nz);
         return lx * ey * ez + ly * ex * ez + lz * ex * ey;
     };
-    double special_cells = 0, lossy_int = 0;
+    double special_cells = 0;
     for (const auto &sp : x->specials) special_cells += (double)sp.cells.size();
-    {
-        // interior cells of lossy materials
-        int64_t n = 0;
-        for (int k = 0; k < x->nz; ++k)
-            for (int j = 1; j < x->ny - 1; ++j)
-                for (int i = 1; i < x->nx - 1; ++i)
-                    if (x->pre[x->medium[x->fidx(i, j, k)]].loss) ++n;
-        lossy_int = (double)n;
-    }
+    const double lossy_int = (double)x->lossy_interior;
     Kernel kh;
     kh.name = "h_update";
     kh.bytes = hcells * 9 * w + pml_inc(nx - 1, ny - 1, nz) * 4 * w;
@@ -513,9 +506,12 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
         x->specials.push_back(std::move(sp));
     }
     for (size_t c = 0; c < x->ncellp; ++c) {
-        if ((int)(c % x->nxp) >= x->nx) continue;
+        const int i = (int)(c % x->nxp);
+        if (i >= x->nx) continue;
         const int so = x->special_of[x->medium[c]];
         if (so >= 0) x->specials[so].cells.push_back((uint32_t)c);
+        const int j = (int)((c / x->nxp) % x->ny);
+        if (x->pre[x->medium[c]].loss && i >= 1 && i <= x->nx - 2 && j >= 1 && j <= x->ny - 2) x->lossy_interior++;
     }
     for (auto &sp : x->specials) {
         const size_t n = sp.cells.size();

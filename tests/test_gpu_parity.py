@@ -55,7 +55,7 @@ def run_case(c, precision):
         if t % out["interval"] == 0:
             sim.run(1)
             slices[t] = sim.slice(out["component"], out["axis"], out["pos"])
-            row = list(sim.point(mon["x"], mon["y"], min(mon["z"] + 1, spec["grid"]["nz"])))
+            row = [t * spec["grid"]["dt"]] + list(sim.point(mon["x"], mon["y"], min(mon["z"] + 1, spec["grid"]["nz"])))
             r, v = sim.ions(mon["x"], mon["y"], mon["z"])
             for a in range(len(r) // 3):
                 row += list(r[3 * a:3 * a + 3]) + list(v[3 * a:3 * a + 3])

@@ -65,3 +65,18 @@ def test_signal_present(case_run):
         assert np.abs(rows[:, 1:10]).max() == 0
     else:
         assert np.abs(rows[:, 1:10]).max() > 0
+
+
+@pytest.mark.parametrize("name", golden_cases.CASES)
+def test_state_roundtrip(name):
+    c = golden_cases.load(name)
+    o = orc.Oracle(c["spec"], c["medium"])
+    for m, mat in enumerate(c["spec"]["materials"]):
+        kinds = ([0, 1] if mat["natom"] else []) + ([2, 3] if mat["use_ade"] else [])
+        for kind in kinds:
+            a = o.state(m, kind)
+            width = 3 * mat["natom"] if kind < 2 else 3
+            assert a.size == int((c["medium"] == m).sum()) * width
+            new = np.arange(a.size, dtype=np.float64)
+            o.set_state(m, kind, new)
+            np.testing.assert_array_equal(o.state(m, kind), new)
