@@ -87,7 +87,7 @@ class Simulation:
     # -- sampling --------------------------------------------------------------
     def slice(self, comp, axis, pos, ref_compat=True):
         nx, ny, nz = self.shape
-        shp = {"z": (nx, ny), "x": (ny, nz), "y": (nx, nz)}[axis]
+        shp = {"z": (nx, ny), "x": (ny, nz), "y": (nx, nz)}.get(axis, (1,))  # bad axes: library says EINVAL
         out = np.empty(shp, np.float32)
         c = _abi.COMPONENTS[comp] if isinstance(comp, str) else comp
         _abi.check(self.lib, self.lib.phpfdtd_read_slice(self.h, c, axis.encode(), pos, 1 if ref_compat else 0,
