@@ -1,0 +1,113 @@
+// bwprobe.hip - plain HBM streaming kernels of known size.  They give the
+// achievable copy bandwidth of the device next to the stencil kernels, and the
+// scale factor of the FETCH_SIZE / WRITE_SIZE counters (the counters' reading
+// for a copy that moves exactly `bytes` in and `bytes` out).
+//   bwprobe [GiB per buffer] [iterations] [variant|all]
+// variants: gs (grid-stride 16 B), u4 (one-shot, 4 x 16 B per thread, loads
+// first), nt (u4 with non-temporal loads/stores), rd (read-only reduction),
+// wr (write-only fill).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        b[q] = a[q];
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void copy_u4(const u32x4 *__restrict__ a, u32x4 *__restrict__ b, size_t n) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t q = base + u * 256;
+        if (q < n) v[u] = NT ? __builtin_nontemporal_load(a + q) : a[q];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t q = base + u * 256;
+        if (q < n) {
+            if (NT) __builtin_nontemporal_store(v[u], b + q);
+            else b[q] = v[u];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void read_u4(const uint4 *__restrict__ a, uint4 *__restrict__ out, size_t n) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    uint4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t q = base + u * 256;
+        if (q < n) {
+            const uint4 v = a[q];
+            acc.x ^= v.x; acc.y ^= v.y; acc.z ^= v.z; acc.w ^= v.w;
+        }
+    }
+    if ((acc.x & 0x7fffffff) == 0x12345678) out[0] = acc;  // keeps the loads alive
+}
+
+__global__ __launch_bounds__(256) void write_u4(uint4 *__restrict__ b, size_t n) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t q = base + u * 256;
+        if (q < n) b[q] = make_uint4(u, 1, 2, 3);
+    }
+}
+
+#define CK(x)                                                                            \
+    do {                                                                                 \
+        hipError_t e = (x);                                                              \
+        if (e != hipSuccess) {                                                           \
+            fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e));                       \
+            return 1;                                                                    \
+        }                                                                                \
+    } while (0)
+
+int main(int argc, char **argv) {
+    const double gib = argc > 1 ? atof(argv[1]) : 4.0;
+    const int iters = argc > 2 ? atoi(argv[2]) : 20;
+    const char *which = argc > 3 ? argv[3] : "gs";
+    const size_t bytes = (size_t)(gib * 1073741824.0) / 4096 * 4096;
+    const size_t n = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    CK(hipMalloc(&a, bytes));
+    CK(hipMalloc(&b, bytes));
+    CK(hipMemset(a, 1, bytes));
+    CK(hipMemset(b, 0, bytes));
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    const char *names[] = {"gs", "u4", "nt", "rd", "wr"};
+    const unsigned one_shot = (unsigned)((n + 1023) / 1024);
+    for (int v = 0; v < 5; ++v) {
+        if (strcmp(which, "all") && strcmp(which, names[v])) continue;
+        for (int it = -1; it < iters; ++it) {
+            if (it == 0) CK(hipEventRecord(e0, 0));
+            switch (v) {
+                case 0: hipLaunchKernelGGL(copy16, dim3(256 * 32), dim3(256), 0, 0, a, b, n); break;
+                case 1: hipLaunchKernelGGL(copy_u4<false>, dim3(one_shot), dim3(256), 0, 0, (const u32x4 *)a, (u32x4 *)b, n); break;
+                case 2: hipLaunchKernelGGL(copy_u4<true>, dim3(one_shot), dim3(256), 0, 0, (const u32x4 *)a, (u32x4 *)b, n); break;
+                case 3: hipLaunchKernelGGL(read_u4, dim3(one_shot), dim3(256), 0, 0, a, b, n); break;
+                case 4: hipLaunchKernelGGL(write_u4, dim3(one_shot), dim3(256), 0, 0, b, n); break;
+            }
+            if (it == -1) CK(hipDeviceSynchronize());
+        }
+        CK(hipEventRecord(e1, 0));
+        CK(hipEventSynchronize(e1));
+        float ms = 0;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / iters;
+        const double moved = (v == 3 || v == 4) ? (double)bytes : 2.0 * bytes;
+        printf("{\"variant\": \"%s\", \"bytes_per_buffer\": %zu, \"moved_bytes\": %.0f, \"ms\": %.4f, \"gbps\": %.1f}\n",
+               names[v], bytes, moved, per, moved / (per * 1e-3) / 1e9);
+    }
+    CK(hipFree(a));
+    CK(hipFree(b));
+    return 0;
+}

@@ -1,8 +1,8 @@
 // kernels_slot.hip - E update of ADE and lattice cells (fdtd.c:3404-3531).
 //
 // Cells of a material that carries ADE polarisation or a phonon lattice are
-// listed once, in x-fastest order, as the slots of that material; one thread
-// updates one slot after the D/E kernel has produced this step's D there.
+// listed once, in x-fastest order, as the slots of that material; the slot
+// kernels run after the D/E kernel has produced this step's D there.
 // Everything is done in the crystal frame with host-precomputed rotation R:
 //   D_cr = R D,  D' = D_cr - P_ion,  P_ion = QeZV r^n           (3417-3427)
 //   ADE:  P^{n+1} = Gp^-1 (2 P^n - Gm P^{n-1} + eps0 dt^2 wp2 R E^n)
@@ -10,9 +10,18 @@
 //   else: E_cr = (1-lt) lf Re(eps^-1) D' / eps0                   (3457-3461)
 //   a = QeZM^T E_cr - DMM r^n;  r += v dt + a dt^2/2;  v += a dt  (3505-3528)
 //   E = R^T E_cr                                                  (3489-3503)
-// The lattice contraction DMM r^n (3N x 3N against the slot's 3N-vector) is
-// evaluated with wavefront-uniform (scalar-cache) operands: DMM, QeZV and
-// QeZM are the same for every slot of the material.
+//
+// Lattice state layout: tiles of 64 slots, r[tile][dof][64] (FP64), so the
+// 3N dofs of the 64 slots a wavefront owns are one contiguous 64*3N*8-byte
+// block; the kernels stream it with non-temporal loads and stores.
+//
+// Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
+// DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
+// Precision form (FP32): one wavefront per 64-slot tile; [DMM; QeZV] r^n is
+// one v_mfma_f32_16x16x4f32 contraction per 16-slot group (A = the stacked
+// 3N+3 x 3N matrix, pre-arranged in lane order on the host; B = r^n rounded
+// to FP32), accumulated in FP32; P_ion falls out of the same product.  The
+// r, v integration stays FP64.
 #include "phpfdtd_internal.h"
 
 namespace phpfdtd {
@@ -31,120 +40,207 @@ __device__ __forceinline__ void rot_t(const real *R, const real x[3], real y[3])
     y[2] = R[2] * x[0] + R[5] * x[1] + R[8] * x[2];
 }
 
-// ADE-only materials (natom == 0).
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+template <typename T>
+__device__ __forceinline__ void stnt(T v, T *p) {
+    __builtin_nontemporal_store(v, p);
+}
+
+// ADE recurrence in the crystal frame for one slot; returns P^{n+1}.
 template <typename real>
-__global__ __launch_bounds__(256) void ade_update(SlotArgs<real> a) {
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= a.nslot) return;
-    const size_t c = a.cells[s];
+__device__ __forceinline__ void ade_step(const SlotArgs<real> &a, int64_t s, const real Eo[3], real Pn[3]) {
     const int64_t n = a.nslot;
-    const real Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
-    const real El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
-    real Dc[3], Ec[3];
-    rot(a.R, Dl, Dc);
-    rot(a.R, El, Ec);
-    real P[3], Po[3], tv[3], Pn[3], En[3], Eo[3];
+    real P[3], Po[3], tv[3];
     for (int e = 0; e < 3; ++e) {
         P[e] = a.P[e * n + s];
         Po[e] = a.Po[e * n + s];
     }
     for (int e = 0; e < 3; ++e)
         tv[e] = 2 * P[e] - (a.Gm[e * 3] * Po[0] + a.Gm[e * 3 + 1] * Po[1] + a.Gm[e * 3 + 2] * Po[2]) +
-                (a.W[e * 3] * Ec[0] + a.W[e * 3 + 1] * Ec[1] + a.W[e * 3 + 2] * Ec[2]);
+                (a.W[e * 3] * Eo[0] + a.W[e * 3 + 1] * Eo[1] + a.W[e * 3 + 2] * Eo[2]);
     for (int e = 0; e < 3; ++e) Pn[e] = a.Gpi[e * 3] * tv[0] + a.Gpi[e * 3 + 1] * tv[1] + a.Gpi[e * 3 + 2] * tv[2];
-    for (int e = 0; e < 3; ++e) Dc[e] -= Pn[e];
-    rot(a.A, Dc, En);
-    rot_t(a.R, En, Eo);
-    a.Ex[c] = Eo[0];
-    a.Ey[c] = Eo[1];
-    a.Ez[c] = Eo[2];
     for (int e = 0; e < 3; ++e) {
         a.Po[e * n + s] = P[e];
         a.P[e * n + s] = Pn[e];
     }
 }
 
-// Lattice materials.  acc_t is the type of the DMM/QeZV/QeZM contraction
-// (double in the plain form, float in the precision form); r, v are FP64.
-// NA > 0 fixes natom at compile time (fully unrolled, r held in registers).
-template <typename real, typename acc_t, int NA>
-__global__ __launch_bounds__(256) void ion_update(SlotArgs<real> a, const acc_t *__restrict__ dmm,
-                                                  const acc_t *__restrict__ qezv,
-                                                  const acc_t *__restrict__ qezm, double *__restrict__ rr,
-                                                  double *__restrict__ vv) {
+// ADE-only materials (natom == 0).
+template <typename real>
+__global__ __launch_bounds__(256) void ade_update(SlotArgs<real> a) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= a.nslot) return;
+    const size_t c = a.cells[s];
+    const real Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
+    const real El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
+    real Dc[3], Ec[3], Pn[3], En[3], Eo[3];
+    rot(a.R, Dl, Dc);
+    rot(a.R, El, Ec);
+    ade_step(a, s, Ec, Pn);
+    for (int e = 0; e < 3; ++e) Dc[e] -= Pn[e];
+    rot(a.A, Dc, En);
+    rot_t(a.R, En, Eo);
+    a.Ex[c] = Eo[0];
+    a.Ey[c] = Eo[1];
+    a.Ez[c] = Eo[2];
+}
+
+// Plain form: one thread per slot, everything FP64.  NA > 0 fixes natom.
+template <int NA>
+__global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const double *__restrict__ dmm,
+                                                      const double *__restrict__ qezv,
+                                                      const double *__restrict__ qezm, double *__restrict__ rr,
+                                                      double *__restrict__ vv) {
     constexpr int NMAX = NA > 0 ? 3 * NA : 3 * kMaxAtom;
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s >= a.nslot) return;
     const int N3 = NA > 0 ? 3 * NA : 3 * a.natom;
     const size_t c = a.cells[s];
-    const int64_t n = a.nslot;
-    const real Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
-    real Dc[3];
-    rot(a.R, Dl, Dc);
-
+    const int64_t tb = (s >> 6) * (int64_t)N3 * 64 + (s & 63);  // AoSoA base of this slot
+    const double Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
     double r[NMAX];
-    acc_t ra[NMAX];
 #pragma unroll
-    for (int q = 0; q < NMAX; ++q) {
-        if (q < N3) {
-            r[q] = rr[q * n + s];
-            ra[q] = (acc_t)r[q];
-        }
-    }
-    // P_ion = QeZV . r  (QeZV[atom][E comp][dir])
-    acc_t P[3] = {0, 0, 0};
+    for (int q = 0; q < NMAX; ++q)
+        if (q < N3) r[q] = ldnt(rr + tb + q * 64);
+    double Dc[3];
+    rot(a.R, Dl, Dc);
+    double P[3] = {0, 0, 0};
 #pragma unroll
     for (int q = 0; q < NMAX; ++q) {
         if (q < N3) {
             const int at = q / 3, d = q % 3;
-            P[0] += qezv[at * 9 + 0 + d] * ra[q];
-            P[1] += qezv[at * 9 + 3 + d] * ra[q];
-            P[2] += qezv[at * 9 + 6 + d] * ra[q];
+            P[0] += qezv[at * 9 + 0 + d] * r[q];
+            P[1] += qezv[at * 9 + 3 + d] * r[q];
+            P[2] += qezv[at * 9 + 6 + d] * r[q];
         }
     }
-    real Dp[3] = {Dc[0] - (real)P[0], Dc[1] - (real)P[1], Dc[2] - (real)P[2]};
-    real Ec[3];
+    double Dp[3] = {Dc[0] - P[0], Dc[1] - P[1], Dc[2] - P[2]};
     if (a.ade) {
-        const real El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
-        real Eo[3], Pp[3], Po[3], tv[3], Pn[3];
+        const double El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
+        double Eo[3], Pn[3];
         rot(a.R, El, Eo);
-        for (int e = 0; e < 3; ++e) {
-            Pp[e] = a.P[e * n + s];
-            Po[e] = a.Po[e * n + s];
-        }
-        for (int e = 0; e < 3; ++e)
-            tv[e] = 2 * Pp[e] - (a.Gm[e * 3] * Po[0] + a.Gm[e * 3 + 1] * Po[1] + a.Gm[e * 3 + 2] * Po[2]) +
-                    (a.W[e * 3] * Eo[0] + a.W[e * 3 + 1] * Eo[1] + a.W[e * 3 + 2] * Eo[2]);
-        for (int e = 0; e < 3; ++e)
-            Pn[e] = a.Gpi[e * 3] * tv[0] + a.Gpi[e * 3 + 1] * tv[1] + a.Gpi[e * 3 + 2] * tv[2];
-        for (int e = 0; e < 3; ++e) {
-            a.Po[e * n + s] = Pp[e];
-            a.P[e * n + s] = Pn[e];
-            Dp[e] -= Pn[e];
-        }
+        ade_step(a, s, Eo, Pn);
+        for (int e = 0; e < 3; ++e) Dp[e] -= Pn[e];
     }
+    double Ec[3], El[3];
     rot(a.A, Dp, Ec);
-    {
-        real El[3];
-        rot_t(a.R, Ec, El);
-        a.Ex[c] = El[0];
-        a.Ey[c] = El[1];
-        a.Ez[c] = El[2];
-    }
-    const acc_t E0 = (acc_t)Ec[0], E1 = (acc_t)Ec[1], E2 = (acc_t)Ec[2];
+    rot_t(a.R, Ec, El);
+    a.Ex[c] = El[0];
+    a.Ey[c] = El[1];
+    a.Ez[c] = El[2];
     const double dt = a.dt, hdt2 = a.half_dt2;
 #pragma unroll
     for (int q = 0; q < NMAX; ++q) {
         if (q < N3) {
             const int at = q / 3, d = q % 3;
-            acc_t acc = qezm[at * 9 + 0 + d] * E0 + qezm[at * 9 + 3 + d] * E1 + qezm[at * 9 + 6 + d] * E2;
+            double acc = qezm[at * 9 + 0 + d] * Ec[0] + qezm[at * 9 + 3 + d] * Ec[1] + qezm[at * 9 + 6 + d] * Ec[2];
 #pragma unroll
             for (int jq = 0; jq < NMAX; ++jq)
-                if (jq < N3) acc -= dmm[q * N3 + jq] * ra[jq];
-            const double aq = (double)acc;
-            const double vq = vv[q * n + s];
-            rr[q * n + s] = r[q] + (vq * dt + aq * hdt2);
-            vv[q * n + s] = vq + aq * dt;
+                if (jq < N3) acc -= dmm[q * N3 + jq] * r[jq];
+            const double vq = ldnt(vv + tb + q * 64);
+            stnt(r[q] + (vq * dt + acc * hdt2), rr + tb + q * 64);
+            stnt(vq + acc * dt, vv + tb + q * 64);
+        }
+    }
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Precision form: one wavefront per 64-slot tile, [DMM; QeZV] r^n on FP32
+// MFMA.  Operand layouts of v_mfma_f32_16x16x4f32 (lane l, gfx950):
+//   A[m][k] in lane m + 16k,  B[k][n] in lane n + 16k,
+//   D[m][n] in lane n + 16 (m / 4), element m % 4.
+// NM m-tiles of 16 rows cover the 3N DMM rows plus the 3 QeZV rows placed at
+// row PR = roundup(3N, 4), so P_ion sits in elements 0..2 of lane group
+// (PR % 16) / 4 of m-tile PR / 16.  NK k-tiles of 4 cover the 3N dofs.
+// aop holds A in lane order: aop[(mt*NK + kt)*64 + l]; qop holds QeZM^T rows
+// in the D layout: qop[((mt*4 + i)*64 + l)*3 + e] = QeZM[m/3][e][m%3].
+template <int NA>
+__global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
+                                                       const float *__restrict__ qop, double *__restrict__ rr,
+                                                       double *__restrict__ vv) {
+    constexpr int NM = ((3 * NA + 3) / 4 * 4 + 3 + 15) / 16;
+    constexpr int NK = (3 * NA + 3) / 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t ntile = (a.nslot + 63) >> 6;
+    if (tile >= ntile) return;  // wave-uniform
+    const int N3 = 3 * a.natom;
+    const int PR = (N3 + 3) & ~3;
+    const int pm = PR >> 4, pg = (PR & 15) >> 2;
+    const int64_t tbase = tile * (int64_t)N3 * 64;
+    float A[NM][NK];
+#pragma unroll
+    for (int mt = 0; mt < NM; ++mt)
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) A[mt][kt] = aop[(mt * NK + kt) * 64 + lane];
+    const double dt = a.dt, hdt2 = a.half_dt2;
+    const int grp = lane >> 4, col = lane & 15;
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        const int sl = 16 * g + col;  // slot within the tile
+        f32x4 acc[NM];
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) {
+            const int dof = 4 * kt + grp;
+            // normal (cached) load: the same lines are re-read below for the update
+            const float b = dof < N3 ? (float)rr[tbase + (int64_t)dof * 64 + sl] : 0.f;
+#pragma unroll
+            for (int mt = 0; mt < NM; ++mt) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b, acc[mt], 0, 0, 0);
+        }
+        // P_ion of slot `col` is in lane group pg of m-tile pm, elements 0..2
+        float P0 = 0.f, P1 = 0.f, P2 = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt)
+            if (mt == pm) { P0 = acc[mt][0]; P1 = acc[mt][1]; P2 = acc[mt][2]; }
+        const int64_t s = tile * 64 + sl;
+        const bool live = s < a.nslot;
+        float Ec[3] = {0.f, 0.f, 0.f};
+        if (grp == pg && live) {
+            const size_t c = a.cells[s];
+            const float Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
+            float Dc[3];
+            rot(a.R, Dl, Dc);
+            float Dp[3] = {Dc[0] - P0, Dc[1] - P1, Dc[2] - P2};
+            if (a.ade) {
+                const float El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
+                float Eo[3], Pn[3];
+                rot(a.R, El, Eo);
+                ade_step(a, s, Eo, Pn);
+                for (int e = 0; e < 3; ++e) Dp[e] -= Pn[e];
+            }
+            rot(a.A, Dp, Ec);
+            float El[3];
+            rot_t(a.R, Ec, El);
+            a.Ex[c] = El[0];
+            a.Ey[c] = El[1];
+            a.Ez[c] = El[2];
+        }
+        // every lane of column `col` needs E_cr of that slot
+        const int src = pg * 16 + col;
+        const float E0 = __shfl(Ec[0], src, 64), E1 = __shfl(Ec[1], src, 64), E2 = __shfl(Ec[2], src, 64);
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * mt + 4 * grp + i;
+                if (m < N3 && live) {
+                    const float *q = qop + ((size_t)(mt * 4 + i) * 64 + lane) * 3;
+                    const float af = q[0] * E0 + q[1] * E1 + q[2] * E2 - acc[mt][i];
+                    const double acc64 = (double)af;
+                    const int64_t idx = tbase + (int64_t)m * 64 + sl;
+                    const double r0 = rr[idx];
+                    const double v0 = ldnt(vv + idx);
+                    stnt(r0 + (v0 * dt + acc64 * hdt2), rr + idx);
+                    stnt(v0 + acc64 * dt, vv + idx);
+                }
+            }
         }
     }
 }
@@ -155,15 +251,17 @@ void launch_ade(const SlotArgs<real> &a, hipStream_t st) {
     if (nb) hipLaunchKernelGGL(ade_update<real>, dim3(nb), dim3(256), 0, st, a);
 }
 
-template <typename real, typename acc_t>
-static void launch_ion_t(const SlotArgs<real> &a, const acc_t *dmm, const acc_t *qezv, const acc_t *qezm,
-                         hipStream_t st) {
+template <typename real>
+void launch_ion(const SlotArgs<real> &a, hipStream_t st);
+
+template <>
+void launch_ion<double>(const SlotArgs<double> &a, hipStream_t st) {
     const unsigned nb = (unsigned)((a.nslot + 255) / 256);
     if (!nb) return;
-#define PHPFDTD_ION_CASE(NN)                                                                      \
-    case NN:                                                                                      \
-        hipLaunchKernelGGL((ion_update<real, acc_t, NN>), dim3(nb), dim3(256), 0, st, a, dmm, qezv, \
-                           qezm, a.r, a.v);                                                       \
+#define PHPFDTD_ION_CASE(NN)                                                                              \
+    case NN:                                                                                              \
+        hipLaunchKernelGGL(ion_update_f64<NN>, dim3(nb), dim3(256), 0, st, a, a.dmm, a.qezv, a.qezm, a.r, \
+                           a.v);                                                                          \
         return;
     switch (a.natom) {
         PHPFDTD_ION_CASE(1)
@@ -175,23 +273,33 @@ static void launch_ion_t(const SlotArgs<real> &a, const acc_t *dmm, const acc_t 
         PHPFDTD_ION_CASE(12)
         PHPFDTD_ION_CASE(16)
         default:
-            hipLaunchKernelGGL((ion_update<real, acc_t, 0>), dim3(nb), dim3(256), 0, st, a, dmm, qezv, qezm,
-                               a.r, a.v);
+            hipLaunchKernelGGL(ion_update_f64<0>, dim3(nb), dim3(256), 0, st, a, a.dmm, a.qezv, a.qezm, a.r, a.v);
     }
 #undef PHPFDTD_ION_CASE
 }
 
-template <typename real>
-void launch_ion(const SlotArgs<real> &a, hipStream_t st);
-
-template <>
-void launch_ion<double>(const SlotArgs<double> &a, hipStream_t st) {
-    launch_ion_t<double, double>(a, a.dmm, a.qezv, a.qezm, st);
-}
-
 template <>
 void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
-    launch_ion_t<float, float>(a, a.dmm32, a.qezv32, a.qezm32, st);
+    const int64_t ntile = (a.nslot + 63) / 64;
+    const unsigned nb = (unsigned)((ntile + 3) / 4);
+    if (!nb) return;
+    // the host lays out aop/qop for the instantiation mfma_inst(natom)
+    switch (mfma_inst(a.natom)) {
+#define PHPFDTD_MFMA_CASE(NN)                                                                                \
+    case NN:                                                                                                 \
+        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3(nb), dim3(256), 0, st, a, a.aop, a.qop, a.r, a.v);     \
+        return;
+        PHPFDTD_MFMA_CASE(1)
+        PHPFDTD_MFMA_CASE(2)
+        PHPFDTD_MFMA_CASE(3)
+        PHPFDTD_MFMA_CASE(4)
+        PHPFDTD_MFMA_CASE(6)
+        PHPFDTD_MFMA_CASE(8)
+        PHPFDTD_MFMA_CASE(12)
+        PHPFDTD_MFMA_CASE(16)
+        PHPFDTD_MFMA_CASE(kMaxAtom)
+#undef PHPFDTD_MFMA_CASE
+    }
 }
 
 template void launch_ade<double>(const SlotArgs<double> &, hipStream_t);

@@ -71,7 +71,7 @@ struct Special {
     void *d_P = nullptr, *d_Po = nullptr;
     double *d_r = nullptr, *d_v = nullptr;
     double *d_dmm = nullptr, *d_qezv = nullptr, *d_qezm = nullptr;
-    float *d_dmm32 = nullptr, *d_qezv32 = nullptr, *d_qezm32 = nullptr;
+    float *d_aop = nullptr, *d_qop = nullptr;  // MFMA operand tables (precision form)
 };
 
 struct Kernel {
@@ -123,8 +123,7 @@ struct phpfdtd_ctx {
     double *d_ring = nullptr;
     int64_t ring_rows = 0, mon_read = 0;
     bool mon_on = false;
-    int64_t mon_c = -1, mon_slot = 0, mon_nslot = 0;
-    int mon_dof = -1;
+    int64_t mon_c = -1, mon_idx = -1;
     const double *mon_r = nullptr, *mon_v = nullptr;
     unsigned long long *d_diag = nullptr;
     float *d_slice = nullptr;
@@ -197,7 +196,8 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
         if (!n) continue;
         l.slots += n * 4;
         if (ade[m]) l.slots += 2 * 3 * n * rs;
-        if (natom[m]) l.slots += 2 * 3 * (size_t)natom[m] * n * 8 + 9 * natom[m] * natom[m] * 12 + 18 * natom[m] * 12;
+        if (natom[m]) l.slots += 2 * 3 * (size_t)natom[m] * ((n + 63) / 64 * 64) * 8 + 9 * natom[m] * natom[m] * 8 +
+                                  18 * natom[m] * 8 + 7 * 24 * 64 * 4 + 7 * 4 * 64 * 3 * 4;
     }
     l.ring = (size_t)g.tsteps * 12 * 8;
     return l;
@@ -224,6 +224,35 @@ static int count_medium(const phpfdtd_grid &g, const int32_t *medium, int nmat, 
         (*cnt)[id]++;
     }
     return 0;
+}
+
+// MFMA operand tables of the precision-form lattice kernel for the
+// instantiation mfma_inst(natom): A = [DMM; 0; QeZV; 0] in lane order
+// (A[m][k] in lane m%16 + 16 (k%4) of tile (m/16, k/4)) and QeZM^T rows in the
+// accumulator layout (row m = 16 mt + 4 (l/16) + i held by lane l, element i).
+static void mfma_tables(const Special &sp, std::vector<float> *aop, std::vector<float> *qop) {
+    const int N = sp.pre.natom, N3 = 3 * N;
+    const int na = mfma_inst(N), NM = mfma_nm(na), NK = mfma_nk(na);
+    const int PR = (N3 + 3) / 4 * 4;
+    auto A = [&](int m, int k) -> double {
+        if (k >= N3) return 0.0;
+        if (m < N3) return sp.dmm[(size_t)m * N3 + k];
+        if (m >= PR && m < PR + 3) return sp.qezv[(k / 3) * 9 + (m - PR) * 3 + (k % 3)];
+        return 0.0;
+    };
+    aop->assign((size_t)NM * NK * 64, 0.f);
+    for (int mt = 0; mt < NM; ++mt)
+        for (int kt = 0; kt < NK; ++kt)
+            for (int l = 0; l < 64; ++l) (*aop)[((size_t)mt * NK + kt) * 64 + l] = (float)A(16 * mt + l % 16, 4 * kt + l / 16);
+    qop->assign((size_t)NM * 4 * 64 * 3, 0.f);
+    for (int mt = 0; mt < NM; ++mt)
+        for (int i = 0; i < 4; ++i)
+            for (int l = 0; l < 64; ++l) {
+                const int m = 16 * mt + 4 * (l / 16) + i;
+                if (m >= N3) continue;
+                for (int e = 0; e < 3; ++e)
+                    (*qop)[(((size_t)mt * 4 + i) * 64 + l) * 3 + e] = (float)sp.qezm[(m / 3) * 9 + e * 3 + (m % 3)];
+            }
 }
 
 template <typename real>
@@ -314,7 +343,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         sa.natom = sp.pre.natom;
         sa.r = sp.d_r; sa.v = sp.d_v;
         sa.dmm = sp.d_dmm; sa.qezv = sp.d_qezv; sa.qezm = sp.d_qezm;
-        sa.dmm32 = sp.d_dmm32; sa.qezv32 = sp.d_qezv32; sa.qezm32 = sp.d_qezm32;
+        sa.aop = sp.d_aop; sa.qop = sp.d_qop;
         sa.dt = x->g.dt;
         sa.half_dt2 = 0.5 * x->g.dt * x->g.dt;
         const double n = (double)sp.cells.size();
@@ -344,9 +373,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
     for (int q = 0; q < 9; ++q) se.F[q] = x->F[q];
     se.mon_r = x->mon_r;
     se.mon_v = x->mon_v;
-    se.mon_slot = x->mon_slot;
-    se.mon_nslot = x->mon_nslot;
-    se.mon_dof = x->mon_dof;
+    se.mon_idx = x->mon_idx;
     se.dt = x->g.dt;
     Kernel ke;
     ke.name = "step_end";
@@ -522,14 +549,18 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
         }
         if (sp.pre.natom > 0) {
             const size_t n3 = 3 * (size_t)sp.pre.natom;
-            if ((rc = x->alloc((void **)&sp.d_r, n3 * n * 8))) return rc;
-            if ((rc = x->alloc((void **)&sp.d_v, n3 * n * 8))) return rc;
+            const size_t npad = (n + 63) / 64 * 64;  // 64-slot tiles
+            if ((rc = x->alloc((void **)&sp.d_r, n3 * npad * 8))) return rc;
+            if ((rc = x->alloc((void **)&sp.d_v, n3 * npad * 8))) return rc;
             if ((rc = x->upload((void **)&sp.d_dmm, sp.dmm))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezv, sp.qezv))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezm, sp.qezm))) return rc;
-            if ((rc = x->upload((void **)&sp.d_dmm32, cast_vec<float>(sp.dmm)))) return rc;
-            if ((rc = x->upload((void **)&sp.d_qezv32, cast_vec<float>(sp.qezv)))) return rc;
-            if ((rc = x->upload((void **)&sp.d_qezm32, cast_vec<float>(sp.qezm)))) return rc;
+            if (x->prec == PHPFDTD_FP32) {
+                std::vector<float> aop, qop;
+                mfma_tables(sp, &aop, &qop);
+                if ((rc = x->upload((void **)&sp.d_aop, aop))) return rc;
+                if ((rc = x->upload((void **)&sp.d_qop, qop))) return rc;
+            }
         }
     }
     if ((rc = x->alloc((void **)&x->d_step, 64))) return rc;
@@ -602,16 +633,14 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
     const int zf = ref_compat ? mz + 1 : mz;
     x->mon_on = true;
     x->mon_c = zf < x->nz ? (int64_t)x->fidx(mx, my, zf) : -1;
-    x->mon_dof = -1;
+    x->mon_idx = -1;
     x->mon_r = x->mon_v = nullptr;
     const int m = x->medium[x->fidx(mx, my, mz)];
     const int sp = x->special_of[m];
     if (sp >= 0 && atom < x->pre[m].natom) {
         int64_t slot = 0;
         find_slot(x, sp, x->fidx(mx, my, mz), &slot);
-        x->mon_dof = atom * 3 + dir;
-        x->mon_slot = slot;
-        x->mon_nslot = (int64_t)x->specials[sp].cells.size();
+        x->mon_idx = ion_index(slot, atom * 3 + dir, 3 * x->pre[m].natom);
         x->mon_r = x->specials[sp].d_r;
         x->mon_v = x->specials[sp].d_v;
     }
@@ -658,7 +687,8 @@ int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
         launch_reduce_e<float>((float *)x->F[0], (float *)x->F[1], (float *)x->F[2], x->nx, x->ny, x->nz, x->nxp,
                                x->d_diag, x->stream);
     for (auto &sp : x->specials)
-        if (sp.pre.natom > 0) launch_reduce_r(sp.d_r, (int64_t)sp.cells.size() * 3 * sp.pre.natom, x->d_diag, x->stream);
+        if (sp.pre.natom > 0)
+            launch_reduce_r(sp.d_r, (int64_t)(sp.cells.size() + 63) / 64 * 64 * 3 * sp.pre.natom, x->d_diag, x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
             launch_argmax_r(sp.d_r, sp.d_cells, (int64_t)sp.cells.size(), 3 * sp.pre.natom, x->ny, x->nz, x->nxp,
@@ -775,8 +805,10 @@ int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double 
     const size_t n = s.cells.size();
     const int n3 = 3 * x->pre[m].natom;
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (r) HIPCHK(hipMemcpy2D(r, 8, s.d_r + slot, n * 8, 8, n3, hipMemcpyDeviceToHost));
-    if (v) HIPCHK(hipMemcpy2D(v, 8, s.d_v + slot, n * 8, 8, n3, hipMemcpyDeviceToHost));
+    const int64_t base = ion_index(slot, 0, n3);  // dof q at base + 64 q
+    (void)n;
+    if (r) HIPCHK(hipMemcpy2D(r, 8, s.d_r + base, 64 * 8, 8, n3, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(hipMemcpy2D(v, 8, s.d_v + base, 64 * 8, 8, n3, hipMemcpyDeviceToHost));
     return x->pre[m].natom;
 }
 
@@ -856,13 +888,15 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
         es = x->rs;
     }
     HIPCHK(hipStreamSynchronize(x->stream));
-    std::vector<unsigned char> h(n * width * es);
+    const bool tiled = kind < 2;  // lattice r, v in 64-slot tiles; ADE P, Po as [3][nslot]
+    const size_t nelem = tiled ? (n + 63) / 64 * 64 * width : n * width;
+    std::vector<unsigned char> h(nelem * es);
     const std::vector<int64_t> ord = file_order(x, s);
-    if (!write) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
+    if (!write || tiled) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < n; ++q) {
         const int64_t slot = ord[q];
         for (int d = 0; d < width; ++d) {
-            const size_t hi = (size_t)d * n + slot;
+            const size_t hi = tiled ? (size_t)ion_index(slot, d, width) : (size_t)d * n + slot;
             double &u = buf[q * width + d];
             if (es == 8) {
                 if (write) ((double *)h.data())[hi] = u;

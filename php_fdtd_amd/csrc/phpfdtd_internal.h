@@ -9,8 +9,9 @@
 //   * material ids: uint8 per cell, same layout.
 //   * CPML psi: compact slabs, not full volumes:
 //       x slabs [nz][ny][2L], y slabs [nz][2L][nxp], z slabs [2L][ny][nxp].
-//   * ADE and lattice state: per-material slot arrays (SoA), slot s of
-//     material m is the s-th cell of m in x-fastest order; r[dof][slot].
+//   * ADE and lattice state: per-material slot arrays, slot s of material m
+//     is the s-th cell of m in x-fastest order; ADE P[3][slot]; lattice r, v
+//     in 64-slot tiles r[tile][dof][64] (ion_index).
 // FP64 host precompute (CPML b/c/kappa, eps^-1, ADE coefficients, rotations,
 // source values) is cast to the working type only when uploaded.
 #pragma once
@@ -86,14 +87,31 @@ struct SlotArgs {
     real *P, *Po;  // [3][nslot]
     // lattice (fdtd.c:3505-3528)
     int natom;
-    double *r, *v;           // [3N][nslot], FP64 state in both forms
+    double *r, *v;           // [tile][3N][64] (64-slot tiles), FP64 state in both forms
     const double *qezv;      // [N][3][3]
     const double *qezm;      // [N][3][3]
     const double *dmm;       // [3N][3N]
-    const float *dmm32;      // FP32 copy [3N][3N] (precision form)
-    const float *qezv32, *qezm32;
+    const float *aop;        // precision form: [DMM; QeZV] MFMA A operands in lane order
+    const float *qop;        // precision form: QeZM^T rows in the MFMA D layout
     double dt, half_dt2;
 };
+
+// Offset of dof q of slot s in a lattice state array (64-slot tiles).
+__host__ __device__ inline int64_t ion_index(int64_t s, int q, int n3) {
+    return (s >> 6) * (int64_t)n3 * 64 + (int64_t)q * 64 + (s & 63);
+}
+
+// natom of the MFMA kernel instantiation used for a lattice of natom atoms,
+// and its m-tile / k-tile counts (16 rows / 4 dofs per tile).  The 3 QeZV rows
+// sit at row roundup(3N, 4) below the 3N DMM rows.
+inline int mfma_inst(int natom) {
+    switch (natom) {
+        case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return natom;
+        default: return kMaxAtom;
+    }
+}
+inline int mfma_nk(int na) { return (3 * na + 3) / 4; }
+inline int mfma_nm(int na) { return ((3 * na + 3) / 4 * 4 + 3 + 15) / 16; }
 
 // Per-step epilogue: monitor row + step counter.
 struct StepEndArgs {
@@ -104,8 +122,7 @@ struct StepEndArgs {
     int64_t mon_c;     // field index sampled (z+1 in reference mode), -1 = zero halo
     const void *F[9];  // field arrays (real)
     const double *mon_r, *mon_v;  // lattice state of the monitor cell's material
-    int64_t mon_slot, mon_nslot;
-    int mon_dof;       // atom*3+dir, -1 = no R/V
+    int64_t mon_idx;   // ion_index of (slot, atom*3+dir), -1 = no R/V
     double dt;
 };
 
