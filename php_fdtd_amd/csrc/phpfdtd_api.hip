@@ -278,14 +278,18 @@ static void build_plan_t(phpfdtd_ctx *x) {
         axes[d]->slot = x->d_slot[d];
     }
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp; a.L = x->L;
+    // a work-group is 4 wavefronts x YeeRows rows x 64 cells; z is split into
+    // chunks so that the launch has at least ~8192 wavefronts (32 per CU)
+    const int rows = kBlockY * YeeRows<real>::value;
     a.nbx = (x->nx + 63) / 64;
-    a.nby = (x->ny + kBlockY - 1) / kBlockY;
-    const int bxy = a.nbx * a.nby;
-    const int target = 2048;
-    int nbz = bxy >= target ? 1 : (target + bxy - 1) / bxy;
+    a.nby = (x->ny + rows - 1) / rows;
+    const int wxy = a.nbx * a.nby * kBlockY;
+    const int target = 8192;
+    int nbz = wxy >= target ? 1 : (target + wxy - 1) / wxy;
     int kc = (x->nz + nbz - 1) / nbz;
-    if (kc < 4) kc = std::min(4, x->nz);
+    if (kc < 8) kc = std::min(8, x->nz);
     nbz = (x->nz + kc - 1) / kc;
+    const int bxy = a.nbx * a.nby;
     a.kchunk = kc;
     a.nbz = nbz;
     a.nblocks = bxy * nbz;

@@ -25,8 +25,13 @@ namespace phpfdtd {
 constexpr int kMaxMat = 64;     // material ids are stored as uint8
 constexpr int kMaxSrc = 8;      // sources handled inside the D/E kernel
 constexpr int kMaxAtom = 32;    // lattice atoms per cell
-constexpr int kBlockY = 4;      // rows per Yee work-group (64 x 4 threads)
+constexpr int kBlockY = 4;      // wavefronts per Yee work-group (64 x 4 threads)
 constexpr double kEpsInf = 20.0;  // EPS_INF of the reference ADE (fdtd.c:202)
+
+// y rows per wavefront in the Yee kernels (register blocking): the same
+// 16 bytes of registers per row-slot in both forms.
+template <typename real> struct YeeRows { static constexpr int value = 8; };
+template <> struct YeeRows<double> { static constexpr int value = 4; };
 
 enum MatFlags : int {
     MF_LOSS = 1,     // eps_im != 0 and no ADE: D -= Lk . E^n (fdtd.c:3349-3353)
