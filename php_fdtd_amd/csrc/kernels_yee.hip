@@ -1,17 +1,21 @@
 // kernels_yee.hip - H update and fused D/E update for gfx950.
 //
-// Work decomposition: a wavefront owns a 64-cell x segment of R consecutive y
-// rows and marches up a chunk of z planes.  Per plane it loads each field row
-// once: the y neighbour of row r is row r+1 (H) or r-1 (D) held in registers,
-// so only one halo row per R rows is fetched twice; the z neighbour (k+1 of E
-// for H, k-1 of H for D) is carried from the previous plane.  The x neighbour
-// (i+1 / i-1) is a load of the same row shifted by one cell, served by the L1
-// from the lines the row load just brought in.  j and k are wavefront-uniform,
-// so every field access is one coalesced row segment.
-//
-// Hardware places work-group b on XCD (b mod 8).  xcd_tile() renumbers the
-// groups so that each XCD owns one contiguous range of tiles; neighbouring
-// tiles then share the XCD's L2.
+// Work decomposition: every lane owns a 16-byte vector of consecutive x cells
+// (4 cells in FP32, 2 in FP64), so a wavefront covers a 1 KiB x segment of a
+// row and every field stream is read and written in 16-byte lane accesses,
+// the width at which the HBM3E streams run fastest on this part.  A
+// wavefront owns R consecutive y rows and marches up a short chunk of z
+// planes:
+//   * z neighbour: E at k+1 (H kernel) / H at k-1 (D kernel) is carried in
+//     registers from the previous plane;
+//   * y neighbour: rows r+1 (H) / r-1 (D) come from registers, one halo row
+//     per R rows is loaded from the L2 / Infinity Cache copy;
+//   * x neighbour: the adjacent cell of the lane's vector, or element 0 (H) /
+//     the last element (D) of the neighbouring lane via a cross-lane shuffle;
+//     only the wavefront's edge lane loads one extra cell.
+// j and k are wavefront-uniform.  Hardware places work-group b on XCD
+// (b mod 8); xcd_tile() renumbers the groups so that each XCD owns one
+// contiguous range of tiles and neighbouring tiles share its L2.
 //
 // Reference semantics (fdtd.c):
 //   H: i < nx-1, j < ny-1, all k; E above the top plane is 0 (3123-3231).
@@ -34,38 +38,69 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks, int nblocks_pad) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <typename T>
-__device__ __forceinline__ T ldnt(const T *p) {
+template <typename real>
+struct Vec;
+template <>
+struct Vec<float> {
+    static constexpr int n = 4;
+    typedef float type __attribute__((ext_vector_type(4)));
+    typedef unsigned int ids;  // 4 material ids
+};
+template <>
+struct Vec<double> {
+    static constexpr int n = 2;
+    typedef double type __attribute__((ext_vector_type(2)));
+    typedef unsigned short ids;  // 2 material ids
+};
+
+template <typename V>
+__device__ __forceinline__ V vld(const V *p) {
     return __builtin_nontemporal_load(p);
 }
-
-template <typename T>
-__device__ __forceinline__ void stnt(T v, T *p) {
+template <typename V>
+__device__ __forceinline__ void vst(V v, V *p) {
     __builtin_nontemporal_store(v, p);
+}
+
+template <typename real>
+__device__ __forceinline__ real shfl_down1(real v) {
+    return __shfl_down(v, 1, 64);
+}
+template <typename real>
+__device__ __forceinline__ real shfl_up1(real v) {
+    return __shfl_up(v, 1, 64);
 }
 
 template <typename real, int R>
 __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
+    constexpr int VN = Vec<real>::n;
+    typedef typename Vec<real>::type V;
     const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
     if (t < 0) return;
     const int bx = t % a.nbx;
     const int rest = t / a.nbx;
     const int by = rest % a.nby;
     const int bz = rest / a.nby;
-    const int i = bx * 64 + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int x0 = (bx * 64 + lane) * VN;  // first cell of this lane
     const int j0 = uniform((by * kBlockY + threadIdx.y) * R);
     const int nr = min(R, a.ny - 1 - j0);  // H rows j < ny-1 of this wavefront
-    if (nr <= 0 || i >= a.nx - 1) return;
+    if (nr <= 0 || uniform(bx * 64 * VN) >= a.nx - 1) return;
+    const bool lane_on = x0 < a.nx - 1;  // lanes past the row end still load and shuffle
+    const int xr = x0 < a.nxp ? x0 : 0;  // in-row offset (lanes past the padded row read cell 0)
     const int k0 = uniform(bz * a.kchunk);
     const int k1 = min(k0 + a.kchunk, a.nz);
     const size_t nxp = a.nxp;
     const size_t plane = nxp * a.ny;
     const int L2 = 2 * a.L;
+    const bool edge = lane == 63;
+    const bool edge_in = x0 + VN < a.nxp;  // the cell after this lane's vector exists
 
-    const real khx = a.ax.kh[i];
-    const int px = a.ax.slot[i];
-    real bxv = 0, chx = 0;
-    if (px >= 0) { bxv = a.ax.b[i]; chx = a.ax.ch[i]; }
+    // per-cell x coefficients (tables padded to nxp)
+    const V khx = *reinterpret_cast<const V *>(a.ax.kh + xr);
+    int px[VN];
+#pragma unroll
+    for (int e = 0; e < VN; ++e) px[e] = x0 + e < a.nx - 1 ? a.ax.slot[xr + e] : -1;
     real khy[R];
     int py[R];
 #pragma unroll
@@ -74,29 +109,31 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
         khy[r] = a.ay.kh[j];
         py[r] = uniform(a.ay.slot[j]);
     }
+    bool anypx = false;
+#pragma unroll
+    for (int e = 0; e < VN; ++e) anypx |= px[e] >= 0;
 
-    // plane k rows j0..j0+nr of Ex (row nr is the y halo), rows j0..j0+nr-1 of Ey
-    size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + i;
-    real ex[R + 1], ey[R];
+    size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + xr;
+    V ex[R + 1], ey[R];
 #pragma unroll
     for (int r = 0; r <= R; ++r)
-        if (r <= nr) ex[r] = a.Ex[c0 + r * nxp];
+        if (r <= nr) ex[r] = *reinterpret_cast<const V *>(a.Ex + c0 + r * nxp);
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (r < nr) ey[r] = a.Ey[c0 + r * nxp];
+        if (r < nr) ey[r] = *reinterpret_cast<const V *>(a.Ey + c0 + r * nxp);
 
     for (int k = k0; k < k1; ++k, c0 += plane) {
-        real exn[R + 1], eyn[R], ez[R + 1];
+        V exn[R + 1], eyn[R], ez[R + 1];
         const bool top = k + 1 >= a.nz;  // E above the top plane is the zero halo
 #pragma unroll
         for (int r = 0; r <= R; ++r)
             if (r <= nr) {
-                exn[r] = top ? real(0) : a.Ex[c0 + plane + r * nxp];
-                ez[r] = a.Ez[c0 + r * nxp];
+                exn[r] = top ? V(0) : *reinterpret_cast<const V *>(a.Ex + c0 + plane + r * nxp);
+                ez[r] = *reinterpret_cast<const V *>(a.Ez + c0 + r * nxp);
             }
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (r < nr) eyn[r] = top ? real(0) : a.Ey[c0 + plane + r * nxp];
+            if (r < nr) eyn[r] = top ? V(0) : *reinterpret_cast<const V *>(a.Ey + c0 + plane + r * nxp);
         const real khz = a.az.kh[k];
         const int pz = uniform(a.az.slot[k]);
         real bzv = 0, chz = 0;
@@ -106,42 +143,80 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
             if (r >= nr) continue;
             const size_t c = c0 + r * nxp;
             const int j = j0 + r;
-            const real ez_ip = a.Ez[c + 1], ey_ip = a.Ey[c + 1];
-            const real dyEz = ez[r + 1] - ez[r], dzEy = eyn[r] - ey[r];
-            const real dxEz = ez_ip - ez[r], dzEx = exn[r] - ex[r];
-            const real dxEy = ey_ip - ey[r], dyEx = ex[r + 1] - ex[r];
-            real hx = ldnt(a.Hx + c), hy = ldnt(a.Hy + c), hz = ldnt(a.Hz + c);
-            hx += khz * dzEy - khy[r] * dyEz;
-            hy += khx * dxEz - khz * dzEx;
-            hz += khy[r] * dyEx - khx * dxEy;
-            if (px >= 0) {
-                const size_t q = ((size_t)k * a.ny + j) * L2 + px;
-                const real sxy = a.pHxy[q], sxz = a.pHxz[q];
-                hy += sxy;
-                hz += sxz;
-                a.pHxy[q] = bxv * sxy + chx * dxEz;
-                a.pHxz[q] = bxv * sxz - chx * dxEy;
+            // x neighbours: element e+1 of the vector, element 0 of the next lane
+            real ez_n = shfl_down1(ez[r][0]), ey_n = shfl_down1(ey[r][0]);
+            if (edge && edge_in) {
+                ez_n = a.Ez[c + VN];
+                ey_n = a.Ey[c + VN];
+            }
+            V ezip, eyip;
+#pragma unroll
+            for (int e = 0; e < VN - 1; ++e) {
+                ezip[e] = ez[r][e + 1];
+                eyip[e] = ey[r][e + 1];
+            }
+            ezip[VN - 1] = ez_n;
+            eyip[VN - 1] = ey_n;
+            const V dyEz = ez[r + 1] - ez[r], dzEy = eyn[r] - ey[r];
+            const V dxEz = ezip - ez[r], dzEx = exn[r] - ex[r];
+            const V dxEy = eyip - ey[r], dyEx = ex[r + 1] - ex[r];
+            V hx = vld(reinterpret_cast<const V *>(a.Hx + c));
+            V hy = vld(reinterpret_cast<const V *>(a.Hy + c));
+            V hz = vld(reinterpret_cast<const V *>(a.Hz + c));
+            V nhx = hx + khz * dzEy - khy[r] * dyEz;
+            V nhy = hy + khx * dxEz - khz * dzEx;
+            V nhz = hz + khy[r] * dyEx - khx * dxEy;
+            if (anypx) {
+#pragma unroll
+                for (int e = 0; e < VN; ++e) {
+                    if (px[e] < 0) continue;
+                    const int i = xr + e;
+                    const size_t q = ((size_t)k * a.ny + j) * L2 + px[e];
+                    const real sxy = a.pHxy[q], sxz = a.pHxz[q];
+                    nhy[e] += sxy;
+                    nhz[e] += sxz;
+                    a.pHxy[q] = a.ax.b[i] * sxy + a.ax.ch[i] * dxEz[e];
+                    a.pHxz[q] = a.ax.b[i] * sxz - a.ax.ch[i] * dxEy[e];
+                }
             }
             if (py[r] >= 0) {
-                const size_t q = ((size_t)k * L2 + py[r]) * nxp + i;
+                V *syxp = reinterpret_cast<V *>(a.pHyx + ((size_t)k * L2 + py[r]) * nxp + xr);
+                V *syzp = reinterpret_cast<V *>(a.pHyz + ((size_t)k * L2 + py[r]) * nxp + xr);
                 const real byv = a.ay.b[j], chy = a.ay.ch[j];
-                const real syx = a.pHyx[q], syz = a.pHyz[q];
-                hx += syx;
-                hz += syz;
-                a.pHyx[q] = byv * syx - chy * dyEz;
-                a.pHyz[q] = byv * syz + chy * dyEx;
+                const V syx = *syxp, syz = *syzp;
+                nhx += syx;
+                nhz += syz;
+                V nsyx = byv * syx - chy * dyEz, nsyz = byv * syz + chy * dyEx;
+#pragma unroll
+                for (int e = 0; e < VN; ++e)
+                    if (x0 + e >= a.nx - 1) { nsyx[e] = syx[e]; nsyz[e] = syz[e]; }
+                if (lane_on) { *syxp = nsyx; *syzp = nsyz; }
             }
             if (pz >= 0) {
-                const size_t q = ((size_t)pz * a.ny + j) * nxp + i;
-                const real szx = a.pHzx[q], szy = a.pHzy[q];
-                hx += szx;
-                hy += szy;
-                a.pHzx[q] = bzv * szx + chz * dzEy;
-                a.pHzy[q] = bzv * szy - chz * dzEx;
+                V *szxp = reinterpret_cast<V *>(a.pHzx + ((size_t)pz * a.ny + j) * nxp + xr);
+                V *szyp = reinterpret_cast<V *>(a.pHzy + ((size_t)pz * a.ny + j) * nxp + xr);
+                const V szx = *szxp, szy = *szyp;
+                nhx += szx;
+                nhy += szy;
+                V nszx = bzv * szx + chz * dzEy, nszy = bzv * szy - chz * dzEx;
+#pragma unroll
+                for (int e = 0; e < VN; ++e)
+                    if (x0 + e >= a.nx - 1) { nszx[e] = szx[e]; nszy[e] = szy[e]; }
+                if (lane_on) { *szxp = nszx; *szyp = nszy; }
             }
-            stnt(hx, a.Hx + c);
-            stnt(hy, a.Hy + c);
-            stnt(hz, a.Hz + c);
+            // cells i >= nx-1 keep their value (never updated by the reference)
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+                if (x0 + e >= a.nx - 1) {
+                    nhx[e] = hx[e];
+                    nhy[e] = hy[e];
+                    nhz[e] = hz[e];
+                }
+            if (lane_on) {
+                vst(nhx, reinterpret_cast<V *>(a.Hx + c));
+                vst(nhy, reinterpret_cast<V *>(a.Hy + c));
+                vst(nhz, reinterpret_cast<V *>(a.Hz + c));
+            }
         }
 #pragma unroll
         for (int r = 0; r <= R; ++r) ex[r] = exn[r];
@@ -152,6 +227,9 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
 
 template <typename real, int R>
 __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
+    constexpr int VN = Vec<real>::n;
+    typedef typename Vec<real>::type V;
+    typedef typename Vec<real>::ids IDS;
     extern __shared__ unsigned char lds_raw[];
     MatCoef<real> *mats = reinterpret_cast<MatCoef<real> *>(lds_raw);
     {
@@ -167,17 +245,32 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
     const int rest = t / a.nbx;
     const int by = rest % a.nby;
     const int bz = rest / a.nby;
-    const int i = bx * 64 + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int x0 = (bx * 64 + lane) * VN;
     const int j0 = uniform((by * kBlockY + threadIdx.y) * R);
     const int nr = min(R, a.ny - j0);  // E/D rows of this wavefront
-    if (nr <= 0 || i >= a.nx) return;
+    if (nr <= 0 || uniform(bx * 64 * VN) >= a.nx) return;
+    const bool lane_on = x0 < a.nx;
+    const int xr = x0 < a.nxp ? x0 : 0;
     const int k0 = uniform(bz * a.kchunk);
     const int k1 = min(k0 + a.kchunk, a.nz);
     const size_t nxp = a.nxp;
     const size_t plane = nxp * a.ny;
     const int L2 = 2 * a.L;
-    const bool inx = i >= 1 && i <= a.nx - 2;
-    // rows of the D loop: 1 <= j <= ny-2 (wavefront-uniform per row)
+    const bool edge = lane == 0 && x0 >= 1;  // needs the cell before its vector from memory
+
+    bool inx[VN];
+    int px[VN];
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+        const int i = x0 + e;
+        inx[e] = i >= 1 && i <= a.nx - 2;
+        px[e] = a.ax.slot[xr + e];
+    }
+    const V kdx = *reinterpret_cast<const V *>(a.ax.kd + xr);
+    bool anypx = false;
+#pragma unroll
+    for (int e = 0; e < VN; ++e) anypx |= px[e] >= 0 && inx[e];
     bool iny[R];
     real kdy[R];
     int py[R];
@@ -189,42 +282,35 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
         kdy[r] = a.ay.kd[jc];
         py[r] = uniform(a.ay.slot[jc]);
     }
-    const bool halo = j0 >= 1;  // row j0-1 exists
-    const real kdx = a.ax.kd[i];
-    const int px = a.ax.slot[i];
-    real bxv = 0, cdx = 0;
-    if (px >= 0) { bxv = a.ax.b[i]; cdx = a.ax.cd[i]; }
+    const bool halo = j0 >= 1;
 
-    // step index selects this step's source values (value*dt), fdtd.c:3363-3396
     const int64_t st = a.nsrc > 0 ? *a.step : 0;
 
-    size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + i;
-    // H at plane k-1 (rows of this wavefront), zero below k = 0
-    real hxm[R], hym[R];
+    size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + xr;
+    V hxm[R], hym[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        hxm[r] = 0;
-        hym[r] = 0;
+        hxm[r] = V(0);
+        hym[r] = V(0);
         if (r < nr && k0 > 0) {
-            hxm[r] = a.Hx[c0 - plane + r * nxp];
-            hym[r] = a.Hy[c0 - plane + r * nxp];
+            hxm[r] = *reinterpret_cast<const V *>(a.Hx + c0 - plane + r * nxp);
+            hym[r] = *reinterpret_cast<const V *>(a.Hy + c0 - plane + r * nxp);
         }
     }
     for (int k = k0; k < k1; ++k, c0 += plane) {
-        // rows -1..nr-1 of Hx and Hz (row -1 is the y halo), rows 0..nr-1 of Hy
-        real hx[R + 1], hz[R + 1], hy[R];
-        hx[0] = 0;
-        hz[0] = 0;
+        V hx[R + 1], hz[R + 1], hy[R];
+        hx[0] = V(0);
+        hz[0] = V(0);
         if (halo) {
-            hx[0] = a.Hx[c0 - nxp];
-            hz[0] = a.Hz[c0 - nxp];
+            hx[0] = *reinterpret_cast<const V *>(a.Hx + c0 - nxp);
+            hz[0] = *reinterpret_cast<const V *>(a.Hz + c0 - nxp);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (r < nr) {
-                hx[r + 1] = a.Hx[c0 + r * nxp];
-                hz[r + 1] = a.Hz[c0 + r * nxp];
-                hy[r] = a.Hy[c0 + r * nxp];
+                hx[r + 1] = *reinterpret_cast<const V *>(a.Hx + c0 + r * nxp);
+                hz[r + 1] = *reinterpret_cast<const V *>(a.Hz + c0 + r * nxp);
+                hy[r] = *reinterpret_cast<const V *>(a.Hy + c0 + r * nxp);
             }
         const real kdz = a.az.kd[k];
         const int pz = uniform(a.az.slot[k]);
@@ -235,66 +321,140 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
             if (r >= nr) continue;
             const size_t c = c0 + r * nxp;
             const int j = j0 + r;
-            real dx = ldnt(a.Dx + c), dy = ldnt(a.Dy + c), dz = ldnt(a.Dz + c);
-            const int m = a.medium[c];
-            const int fl = mats[m].flags;
-            if (inx && iny[r]) {
-                const real hy_im = a.Hy[c - 1], hz_im = a.Hz[c - 1];
-                const real dyHz = hz[r + 1] - hz[r], dzHy = hy[r] - hym[r];
-                const real dzHx = hx[r + 1] - hxm[r], dxHz = hz[r + 1] - hz_im;
-                const real dxHy = hy[r] - hy_im, dyHx = hx[r + 1] - hx[r];
-                dx += kdy[r] * dyHz - kdz * dzHy;
-                dy += kdz * dzHx - kdx * dxHz;
-                dz += kdx * dxHy - kdy[r] * dyHx;
-                if (px >= 0) {
-                    const size_t q = ((size_t)k * a.ny + j) * L2 + px;
-                    const real sxy = a.pDxy[q], sxz = a.pDxz[q];
-                    dy += sxy;
-                    dz += sxz;
-                    a.pDxy[q] = bxv * sxy - cdx * dxHz;
-                    a.pDxz[q] = bxv * sxz + cdx * dxHy;
+            V dx = vld(reinterpret_cast<const V *>(a.Dx + c));
+            V dy = vld(reinterpret_cast<const V *>(a.Dy + c));
+            V dz = vld(reinterpret_cast<const V *>(a.Dz + c));
+            const IDS idw = *reinterpret_cast<const IDS *>(a.medium + c);
+            int mid[VN], fl[VN], anyloss = 0, allplain = 1;
+#pragma unroll
+            for (int e = 0; e < VN; ++e) {
+                mid[e] = (idw >> (8 * e)) & 0xff;
+                fl[e] = mats[mid[e]].flags;
+                anyloss |= fl[e] & MF_LOSS;
+                allplain &= !(fl[e] & MF_SPECIAL);
+            }
+            // x neighbours at i-1: element e-1, or the last element of the previous lane
+            real hy_p = shfl_up1(hy[r][VN - 1]), hz_p = shfl_up1(hz[r + 1][VN - 1]);
+            if (edge) {
+                hy_p = a.Hy[c - 1];
+                hz_p = a.Hz[c - 1];
+            }
+            V hyim, hzim;
+            hyim[0] = hy_p;
+            hzim[0] = hz_p;
+#pragma unroll
+            for (int e = 1; e < VN; ++e) {
+                hyim[e] = hy[r][e - 1];
+                hzim[e] = hz[r + 1][e - 1];
+            }
+            if (iny[r]) {
+                const V dyHz = hz[r + 1] - hz[r], dzHy = hy[r] - hym[r];
+                const V dzHx = hx[r + 1] - hxm[r], dxHz = hz[r + 1] - hzim;
+                const V dxHy = hy[r] - hyim, dyHx = hx[r + 1] - hx[r];
+                V ndx = dx + kdy[r] * dyHz - kdz * dzHy;
+                V ndy = dy + kdz * dzHx - kdx * dxHz;
+                V ndz = dz + kdx * dxHy - kdy[r] * dyHx;
+                if (anypx) {
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) {
+                        if (px[e] < 0 || !inx[e]) continue;
+                        const int i = xr + e;
+                        const size_t q = ((size_t)k * a.ny + j) * L2 + px[e];
+                        const real sxy = a.pDxy[q], sxz = a.pDxz[q];
+                        ndy[e] += sxy;
+                        ndz[e] += sxz;
+                        a.pDxy[q] = a.ax.b[i] * sxy - a.ax.cd[i] * dxHz[e];
+                        a.pDxz[q] = a.ax.b[i] * sxz + a.ax.cd[i] * dxHy[e];
+                    }
                 }
                 if (py[r] >= 0) {
-                    const size_t q = ((size_t)k * L2 + py[r]) * nxp + i;
+                    V *syxp = reinterpret_cast<V *>(a.pDyx + ((size_t)k * L2 + py[r]) * nxp + xr);
+                    V *syzp = reinterpret_cast<V *>(a.pDyz + ((size_t)k * L2 + py[r]) * nxp + xr);
                     const real byv = a.ay.b[j], cdy = a.ay.cd[j];
-                    const real syx = a.pDyx[q], syz = a.pDyz[q];
-                    dx += syx;
-                    dz += syz;
-                    a.pDyx[q] = byv * syx + cdy * dyHz;
-                    a.pDyz[q] = byv * syz - cdy * dyHx;
+                    const V syx = *syxp, syz = *syzp;
+                    ndx += syx;
+                    ndz += syz;
+                    // psi of cells outside the D loop (i = 0, nx-1) is never advanced
+                    V nsyx = byv * syx + cdy * dyHz, nsyz = byv * syz - cdy * dyHx;
+#pragma unroll
+                    for (int e = 0; e < VN; ++e)
+                        if (!inx[e]) { nsyx[e] = syx[e]; nsyz[e] = syz[e]; }
+                    if (lane_on) { *syxp = nsyx; *syzp = nsyz; }
                 }
                 if (pz >= 0) {
-                    const size_t q = ((size_t)pz * a.ny + j) * nxp + i;
-                    const real szx = a.pDzx[q], szy = a.pDzy[q];
-                    dx += szx;
-                    dy += szy;
-                    a.pDzx[q] = bzv * szx - cdz * dzHy;
-                    a.pDzy[q] = bzv * szy + cdz * dzHx;
+                    V *szxp = reinterpret_cast<V *>(a.pDzx + ((size_t)pz * a.ny + j) * nxp + xr);
+                    V *szyp = reinterpret_cast<V *>(a.pDzy + ((size_t)pz * a.ny + j) * nxp + xr);
+                    const V szx = *szxp, szy = *szyp;
+                    ndx += szx;
+                    ndy += szy;
+                    V nszx = bzv * szx - cdz * dzHy, nszy = bzv * szy + cdz * dzHx;
+#pragma unroll
+                    for (int e = 0; e < VN; ++e)
+                        if (!inx[e]) { nszx[e] = szx[e]; nszy[e] = szy[e]; }
+                    if (lane_on) { *szxp = nszx; *szyp = nszy; }
                 }
-                if (fl & MF_LOSS) {
-                    const real ex = a.Ex[c], ey = a.Ey[c], ez = a.Ez[c];
-                    const real *Lk = mats[m].Lk;
-                    dx -= Lk[0] * ex + Lk[1] * ey + Lk[2] * ez;
-                    dy -= Lk[3] * ex + Lk[4] * ey + Lk[5] * ez;
-                    dz -= Lk[6] * ex + Lk[7] * ey + Lk[8] * ez;
+                if (anyloss) {
+                    const V ex = *reinterpret_cast<const V *>(a.Ex + c);
+                    const V ey = *reinterpret_cast<const V *>(a.Ey + c);
+                    const V ez = *reinterpret_cast<const V *>(a.Ez + c);
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) {
+                        if (!(fl[e] & MF_LOSS)) continue;
+                        const real *Lk = mats[mid[e]].Lk;
+                        ndx[e] -= Lk[0] * ex[e] + Lk[1] * ey[e] + Lk[2] * ez[e];
+                        ndy[e] -= Lk[3] * ex[e] + Lk[4] * ey[e] + Lk[5] * ez[e];
+                        ndz[e] -= Lk[6] * ex[e] + Lk[7] * ey[e] + Lk[8] * ez[e];
+                    }
                 }
+#pragma unroll
+                for (int e = 0; e < VN; ++e)
+                    if (inx[e]) {
+                        dx[e] = ndx[e];
+                        dy[e] = ndy[e];
+                        dz[e] = ndz[e];
+                    }
             }
             for (int s = 0; s < a.nsrc; ++s) {
-                if (a.src_k[s] == k && a.src_j[s] == j && a.src_i[s] == i) {
-                    const real v = a.src_val[st * a.nsrc + s];
-                    if (a.src_c[s] == 0) dx += v;
-                    else if (a.src_c[s] == 1) dy += v;
-                    else dz += v;
+                if (a.src_k[s] == k && a.src_j[s] == j) {
+                    const int e = a.src_i[s] - x0;
+                    if (e >= 0 && e < VN) {
+                        const real v = a.src_val[st * a.nsrc + s];
+#pragma unroll
+                        for (int q = 0; q < VN; ++q)
+                            if (q == e) {
+                                if (a.src_c[s] == 0) dx[q] += v;
+                                else if (a.src_c[s] == 1) dy[q] += v;
+                                else dz[q] += v;
+                            }
+                    }
                 }
             }
-            stnt(dx, a.Dx + c);
-            stnt(dy, a.Dy + c);
-            stnt(dz, a.Dz + c);
-            if (!(fl & MF_SPECIAL)) {
-                const real *M = mats[m].M;
-                stnt(M[0] * dx + M[1] * dy + M[2] * dz, a.Ex + c);
-                stnt(M[3] * dx + M[4] * dy + M[5] * dz, a.Ey + c);
-                stnt(M[6] * dx + M[7] * dy + M[8] * dz, a.Ez + c);
+            if (lane_on) {
+                vst(dx, reinterpret_cast<V *>(a.Dx + c));
+                vst(dy, reinterpret_cast<V *>(a.Dy + c));
+                vst(dz, reinterpret_cast<V *>(a.Dz + c));
+                if (allplain) {
+                    V nex, ney, nez;
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) {
+                        const real *M = mats[mid[e]].M;
+                        nex[e] = M[0] * dx[e] + M[1] * dy[e] + M[2] * dz[e];
+                        ney[e] = M[3] * dx[e] + M[4] * dy[e] + M[5] * dz[e];
+                        nez[e] = M[6] * dx[e] + M[7] * dy[e] + M[8] * dz[e];
+                    }
+                    vst(nex, reinterpret_cast<V *>(a.Ex + c));
+                    vst(ney, reinterpret_cast<V *>(a.Ey + c));
+                    vst(nez, reinterpret_cast<V *>(a.Ez + c));
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) {
+                        if (fl[e] & MF_SPECIAL) continue;
+                        const real *M = mats[mid[e]].M;
+                        a.Ex[c + e] = M[0] * dx[e] + M[1] * dy[e] + M[2] * dz[e];
+                        a.Ey[c + e] = M[3] * dx[e] + M[4] * dy[e] + M[5] * dz[e];
+                        a.Ez[c + e] = M[6] * dx[e] + M[7] * dy[e] + M[8] * dz[e];
+                    }
+                }
             }
         }
 #pragma unroll
@@ -305,16 +465,33 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
     }
 }
 
+template <typename real, int R>
+static void launch_h_r(const YeeArgs<real> &a, hipStream_t s) {
+    hipLaunchKernelGGL((h_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), 0, s, a);
+}
+
+template <typename real, int R>
+static void launch_de_r(const YeeArgs<real> &a, int nmat, hipStream_t s) {
+    const size_t lds = (size_t)nmat * sizeof(MatCoef<real>);
+    hipLaunchKernelGGL((de_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a, nmat);
+}
+
 template <typename real>
 void launch_h(const YeeArgs<real> &a, hipStream_t s) {
-    hipLaunchKernelGGL((h_update<real, YeeRows<real>::value>), dim3(a.nblocks_pad), dim3(64, kBlockY), 0, s, a);
+    switch (a.rows) {
+        case 1: launch_h_r<real, 1>(a, s); break;
+        case 2: launch_h_r<real, 2>(a, s); break;
+        default: launch_h_r<real, 4>(a, s); break;
+    }
 }
 
 template <typename real>
 void launch_de(const YeeArgs<real> &a, int nmat, hipStream_t s) {
-    const size_t lds = (size_t)nmat * sizeof(MatCoef<real>);
-    hipLaunchKernelGGL((de_update<real, YeeRows<real>::value>), dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a,
-                       nmat);
+    switch (a.rows) {
+        case 1: launch_de_r<real, 1>(a, nmat, s); break;
+        case 2: launch_de_r<real, 2>(a, nmat, s); break;
+        default: launch_de_r<real, 4>(a, nmat, s); break;
+    }
 }
 
 template void launch_h<double>(const YeeArgs<double> &, hipStream_t);

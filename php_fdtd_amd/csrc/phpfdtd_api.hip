@@ -113,6 +113,7 @@ struct phpfdtd_ctx {
     std::vector<Special> specials;
     int special_of[kMaxMat];
     int64_t lossy_interior = 0;  // interior cells of materials with eps_im loss
+    int yee_rows = 2, yee_kchunk = 8;  // Yee tile: y rows per wavefront, z planes per chunk
     // sources
     int nsrc = 0;
     int src_i[kMaxSrc] = {}, src_j[kMaxSrc] = {}, src_k[kMaxSrc] = {}, src_c[kMaxSrc] = {};
@@ -278,17 +279,15 @@ static void build_plan_t(phpfdtd_ctx *x) {
         axes[d]->slot = x->d_slot[d];
     }
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp; a.L = x->L;
-    // a work-group is 4 wavefronts x YeeRows rows x 64 cells; z is split into
-    // chunks so that the launch has at least ~8192 wavefronts (32 per CU)
-    const int rows = kBlockY * YeeRows<real>::value;
-    a.nbx = (x->nx + 63) / 64;
-    a.nby = (x->ny + rows - 1) / rows;
-    const int wxy = a.nbx * a.nby * kBlockY;
-    const int target = 8192;
-    int nbz = wxy >= target ? 1 : (target + wxy - 1) / wxy;
-    int kc = (x->nz + nbz - 1) / nbz;
-    if (kc < 8) kc = std::min(8, x->nz);
-    nbz = (x->nz + kc - 1) / kc;
+    // a work-group is 4 wavefronts, each `rows` y rows x 64 lanes x one
+    // 16-byte vector of x cells, marching over a short chunk of z planes
+    // (streams of many short-lived wavefronts run closest to the HBM peak)
+    a.rows = x->yee_rows;
+    const int vec = YeeVec<real>::value;
+    a.nbx = (x->nx + 64 * vec - 1) / (64 * vec);
+    a.nby = (x->ny + kBlockY * a.rows - 1) / (kBlockY * a.rows);
+    int kc = std::min(x->yee_kchunk, x->nz);
+    const int nbz = (x->nz + kc - 1) / kc;
     const int bxy = a.nbx * a.nby;
     a.kchunk = kc;
     a.nbz = nbz;
@@ -467,6 +466,15 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     x->nmat = nmat;
     const char *ng = getenv("PHPFDTD_NO_GRAPH");
     x->use_graph = !(ng && ng[0] == '1');
+    // Yee tile shape (defaults measured on MI355X); overridable for tuning
+    if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
+        const int v = atoi(e);
+        if (v == 1 || v == 2 || v == 4) x->yee_rows = v;
+    }
+    if (const char *e = getenv("PHPFDTD_YEE_KCHUNK")) {
+        const int v = atoi(e);
+        if (v >= 1) x->yee_kchunk = v;
+    }
     x->pre.resize(nmat);
     for (int m = 0; m < nmat; ++m) {
         std::string err;
@@ -514,6 +522,14 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     const double ds[3] = {g->dx, g->dy, g->dz};
     for (int d = 0; d < 3; ++d) {
         AxisHost ax = axis_tables(ns[d], ds[d], *g);
+        if (d == 0) {  // x tables are read as 16-byte vectors over the padded row
+            ax.b.resize(x->nxp, 0.0);
+            ax.kh.resize(x->nxp, 0.0);
+            ax.kd.resize(x->nxp, 0.0);
+            ax.ch.resize(x->nxp, 0.0);
+            ax.cd.resize(x->nxp, 0.0);
+            ax.slot.resize(x->nxp, -1);
+        }
         const std::vector<double> *tabs[5] = {&ax.b, &ax.kh, &ax.kd, &ax.ch, &ax.cd};
         for (int t = 0; t < 5; ++t)
             if ((rc = x->upload_real(&x->d_axis[d][t], *tabs[t]))) return rc;

@@ -4,8 +4,8 @@
 // Device layout (HBM), chosen for gfx950 rather than copied from fdtd.c:
 //   * fields Ex..Hz: one array per component, x fastest, row pitch nxp
 //     (nx rounded up to 32 elements), cell (i,j,k) at (k*ny + j)*nxp + i.  A
-//     64-lane wavefront reads 64 consecutive x cells of one row, so every
-//     field stream is a fully coalesced 256 B (FP32) / 512 B (FP64) access.
+//     lane of the Yee kernels reads a 16-byte vector of consecutive x cells,
+//     a 64-lane wavefront a 1 KiB row segment.
 //   * material ids: uint8 per cell, same layout.
 //   * CPML psi: compact slabs, not full volumes:
 //       x slabs [nz][ny][2L], y slabs [nz][2L][nxp], z slabs [2L][ny][nxp].
@@ -28,10 +28,8 @@ constexpr int kMaxAtom = 32;    // lattice atoms per cell
 constexpr int kBlockY = 4;      // wavefronts per Yee work-group (64 x 4 threads)
 constexpr double kEpsInf = 20.0;  // EPS_INF of the reference ADE (fdtd.c:202)
 
-// y rows per wavefront in the Yee kernels (register blocking): the same
-// 16 bytes of registers per row-slot in both forms.
-template <typename real> struct YeeRows { static constexpr int value = 8; };
-template <> struct YeeRows<double> { static constexpr int value = 4; };
+// x cells per lane in the Yee kernels: one 16-byte vector.
+template <typename real> struct YeeVec { static constexpr int value = 16 / sizeof(real); };
 
 enum MatFlags : int {
     MF_LOSS = 1,     // eps_im != 0 and no ADE: D -= Lk . E^n (fdtd.c:3349-3353)
@@ -68,6 +66,7 @@ struct YeeArgs {
     const MatCoef<real> *mats;
     Axis<real> ax, ay, az;
     int nx, ny, nz, nxp, L;
+    int rows;  // y rows per wavefront (1, 2 or 4)
     int kchunk, nbx, nby, nbz, nblocks, nblocks_pad;
     // sources: value*dt per step in src_val[step*nsrc + s]
     int nsrc;
