@@ -15,9 +15,9 @@ __global__ void step_end(StepEndArgs a) {
         row[0] = (double)st * a.dt;
         for (int q = 0; q < 9; ++q)
             row[1 + q] = a.mon_c >= 0 ? (double)static_cast<const real *>(a.F[q])[a.mon_c] : 0.0;
-        if (a.mon_idx >= 0) {
-            row[10] = a.mon_r[a.mon_idx];
-            row[11] = a.mon_v[a.mon_idx];
+        if (a.mon_ridx >= 0) {
+            row[10] = a.mon_r[a.mon_ridx];
+            row[11] = a.mon_v[a.mon_vidx];
         } else {
             row[10] = 0.0;
             row[11] = 0.0;
@@ -77,18 +77,18 @@ __global__ __launch_bounds__(256) void reduce_r(const double *r, int64_t n, unsi
 
 // First argmax in the reference's loop order (i, j, k, atom, dir): among the
 // dofs equal to the maximum, the smallest key ((i*ny + j)*nz + k)*96 + q.
-// r is in 64-slot tiles, r[tile][dof][64].
-__global__ __launch_bounds__(256) void argmax_r(const double *r, const uint32_t *cells, int64_t nslot, int n3,
-                                                int ny, int nz, int nxp, unsigned long long *out) {
+// r is in the material's lattice layout (IonLayout).
+__global__ __launch_bounds__(256) void argmax_r(const double *r, const uint32_t *cells, int64_t nslot,
+                                                IonLayout lay, int ny, int nz, int nxp, unsigned long long *out) {
     const unsigned long long target = out[2];
-    const int64_t total = (nslot + 63) / 64 * 64 * n3;
+    const int64_t total = (nslot + 63) / 64 * lay.tile_r();
     unsigned long long best = ~0ull;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
         if ((unsigned long long)__double_as_longlong(fabs(r[q])) != target) continue;
-        const int64_t tile = q / ((int64_t)n3 * 64), rem = q % ((int64_t)n3 * 64);
-        const int64_t dof = rem / 64, s = tile * 64 + rem % 64;
-        if (s >= nslot) continue;
+        int64_t s;
+        int dof;
+        if (!lay.r_decode(q, nslot, &s, &dof)) continue;
         const int64_t c = cells[s];
         const int64_t i = c % nxp, j = (c / nxp) % ny, k = c / ((int64_t)nxp * ny);
         const unsigned long long key = (unsigned long long)(((i * ny + j) * nz + k) * 96 + dof);
@@ -154,9 +154,9 @@ void launch_reduce_r(const double *r, int64_t n, unsigned long long *out, hipStr
     if (n > 0) hipLaunchKernelGGL(reduce_r, dim3(1024), dim3(256), 0, s, r, n, out);
 }
 
-void launch_argmax_r(const double *r, const uint32_t *cells, int64_t nslot, int n3, int ny, int nz, int nxp,
+void launch_argmax_r(const double *r, const uint32_t *cells, int64_t nslot, IonLayout lay, int ny, int nz, int nxp,
                      unsigned long long *out, hipStream_t s) {
-    if (nslot > 0) hipLaunchKernelGGL(argmax_r, dim3(1024), dim3(256), 0, s, r, cells, nslot, n3, ny, nz, nxp, out);
+    if (nslot > 0) hipLaunchKernelGGL(argmax_r, dim3(1024), dim3(256), 0, s, r, cells, nslot, lay, ny, nz, nxp, out);
 }
 
 template <typename real>

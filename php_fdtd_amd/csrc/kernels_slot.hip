@@ -11,9 +11,9 @@
 //   a = QeZM^T E_cr - DMM r^n;  r += v dt + a dt^2/2;  v += a dt  (3505-3528)
 //   E = R^T E_cr                                                  (3489-3503)
 //
-// Lattice state layout: tiles of 64 slots, r[tile][dof][64] (FP64), so the
-// 3N dofs of the 64 slots a wavefront owns are one contiguous 64*3N*8-byte
-// block; the kernels stream it with non-temporal loads and stores.
+// Lattice state layout (IonLayout): tiles of 64 slots; the plain form keeps
+// r[tile][dof][64], the precision form the MFMA operand orders, so every
+// wavefront access is 512 contiguous bytes streamed non-temporally.
 //
 // Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
 // DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const 
     if (s >= a.nslot) return;
     const int N3 = NA > 0 ? 3 * NA : 3 * a.natom;
     const size_t c = a.cells[s];
-    const int64_t tb = (s >> 6) * (int64_t)N3 * 64 + (s & 63);  // AoSoA base of this slot
+    const int64_t tb = a.lay.r_index(s, 0);  // dof q at tb + 64 q (plain layout, same for v)
     const double Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
     double r[NMAX];
 #pragma unroll
@@ -150,95 +150,137 @@ __global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const 
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Precision form: one wavefront per 64-slot tile, [DMM; QeZV] r^n on FP32
-// MFMA.  Operand layouts of v_mfma_f32_16x16x4f32 (lane l, gfx950):
+// Precision form: one wavefront per 64-slot tile, on FP32 MFMA.  Operand
+// layouts of v_mfma_f32_16x16x4f32 (lane l, gfx950):
 //   A[m][k] in lane m + 16k,  B[k][n] in lane n + 16k,
 //   D[m][n] in lane n + 16 (m / 4), element m % 4.
-// NM m-tiles of 16 rows cover the 3N DMM rows plus the 3 QeZV rows placed at
-// row PR = roundup(3N, 4), so P_ion sits in elements 0..2 of lane group
-// (PR % 16) / 4 of m-tile PR / 16.  NK k-tiles of 4 cover the 3N dofs.
-// aop holds A in lane order: aop[(mt*NK + kt)*64 + l]; qop holds QeZM^T rows
-// in the D layout: qop[((mt*4 + i)*64 + l)*3 + e] = QeZM[m/3][e][m%3].
+// Per 16-slot group g of the tile:
+//   1. acc = [-DMM; 0; QeZV] . r^n over NK k-tiles (B = r^n rounded to FP32,
+//      read straight from the B-ordered state); the QeZV rows at
+//      PR = roundup(3N, 4) give P_ion in elements 0..2 of lane group
+//      (PR % 16) / 4 of m-tile PR / 16.
+//   2. lane 16g + n (which prefetched D of slot 16g + n) forms E_cr of its
+//      slot from P_ion; E_cr is spread as a 4th-operand k-tile (E component
+//      k in lane n + 16k) and acc += QeZM^T . E_cr, one MFMA per m-tile, so
+//      acc = QeZM^T E_cr - DMM r^n = a.
+//   3. each lane integrates the dofs its accumulator holds in FP64: v from
+//      the accumulator-ordered state, r re-read from the cache lines step 1
+//      just brought in.
+// aop[(mt*nk + kt)*64 + l] and qop[mt*64 + l] hold the A operands in lane order.
 template <int NA>
 __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
                                                        const float *__restrict__ qop, double *__restrict__ rr,
                                                        double *__restrict__ vv) {
-    constexpr int NM = ((3 * NA + 3) / 4 * 4 + 3 + 15) / 16;
-    constexpr int NK = (3 * NA + 3) / 4;
+    constexpr int NMX = ((3 * NA + 3) / 4 * 4 + 3 + 15) / 16;
+    constexpr int NKX = (3 * NA + 3) / 4;
+    constexpr int NAX = (3 * NA + 15) / 16;
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t ntile = (a.nslot + 63) >> 6;
     if (tile >= ntile) return;  // wave-uniform
-    const int N3 = 3 * a.natom;
+    const IonLayout L = a.lay;
+    // exact instantiations (natom == NA) use compile-time tile counts, so every
+    // operand array is fully unrolled into registers; the generic one (NA =
+    // kMaxAtom) runs to the material's own counts
+    constexpr bool exact = NA != kMaxAtom;
+    const int N3 = exact ? 3 * NA : L.n3;
+    const int nk = exact ? NKX : L.nk, nma = exact ? NAX : L.nma;
+    const int nm = exact ? NMX : ((N3 + 3) / 4 * 4 + 3 + 15) / 16;
     const int PR = (N3 + 3) & ~3;
     const int pm = PR >> 4, pg = (PR & 15) >> 2;
-    const int64_t tbase = tile * (int64_t)N3 * 64;
-    float A[NM][NK];
-#pragma unroll
-    for (int mt = 0; mt < NM; ++mt)
-#pragma unroll
-        for (int kt = 0; kt < NK; ++kt) A[mt][kt] = aop[(mt * NK + kt) * 64 + lane];
-    const double dt = a.dt, hdt2 = a.half_dt2;
     const int grp = lane >> 4, col = lane & 15;
+    const int64_t rbase = tile * L.tile_r(), vbase = tile * L.tile_v();
+
+    float A[NMX][NKX], AQ[NAX];
+#pragma unroll
+    for (int mt = 0; mt < NMX; ++mt)
+#pragma unroll
+        for (int kt = 0; kt < NKX; ++kt) A[mt][kt] = (mt < nm && kt < nk) ? aop[(mt * nk + kt) * 64 + lane] : 0.f;
+#pragma unroll
+    for (int mt = 0; mt < NAX; ++mt) AQ[mt] = mt < nma ? qop[mt * 64 + lane] : 0.f;
+
+    // lane l prefetches D (and E^n for ADE) of slot 64 tile + l
+    const int64_t my_slot = tile * 64 + lane;
+    const bool my_live = my_slot < a.nslot;
+    size_t my_cell = 0;
+    float Dl[3] = {0.f, 0.f, 0.f}, Eold[3] = {0.f, 0.f, 0.f};
+    if (my_live) {
+        my_cell = a.cells[my_slot];
+        Dl[0] = a.Dx[my_cell];
+        Dl[1] = a.Dy[my_cell];
+        Dl[2] = a.Dz[my_cell];
+        if (a.ade) {
+            Eold[0] = a.Ex[my_cell];
+            Eold[1] = a.Ey[my_cell];
+            Eold[2] = a.Ez[my_cell];
+        }
+    }
+    const double dt = a.dt, hdt2 = a.half_dt2;
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
-        const int sl = 16 * g + col;  // slot within the tile
-        f32x4 acc[NM];
+        f32x4 acc[NMX];
 #pragma unroll
-        for (int mt = 0; mt < NM; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        for (int mt = 0; mt < NMX; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < NK; ++kt) {
-            const int dof = 4 * kt + grp;
-            // normal (cached) load: the same lines are re-read below for the update
-            const float b = dof < N3 ? (float)rr[tbase + (int64_t)dof * 64 + sl] : 0.f;
+        for (int kt = 0; kt < NKX; ++kt) {
+            if (kt >= nk) break;
+            // cached load: step 3 re-reads these lines for the FP64 update
+            const float b = (float)rr[rbase + ((int64_t)g * nk + kt) * 64 + lane];
 #pragma unroll
-            for (int mt = 0; mt < NM; ++mt) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b, acc[mt], 0, 0, 0);
+            for (int mt = 0; mt < NMX; ++mt)
+                if (mt < nm) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b, acc[mt], 0, 0, 0);
         }
-        // P_ion of slot `col` is in lane group pg of m-tile pm, elements 0..2
-        float P0 = 0.f, P1 = 0.f, P2 = 0.f;
+        // P_ion of slot 16g + col to lane 16g + col
+        float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll
-        for (int mt = 0; mt < NM; ++mt)
-            if (mt == pm) { P0 = acc[mt][0]; P1 = acc[mt][1]; P2 = acc[mt][2]; }
-        const int64_t s = tile * 64 + sl;
-        const bool live = s < a.nslot;
+        for (int mt = 0; mt < NMX; ++mt)
+            if (mt == pm) { p0 = acc[mt][0]; p1 = acc[mt][1]; p2 = acc[mt][2]; }
+        const int psrc = pg * 16 + col;
+        p0 = __shfl(p0, psrc, 64);
+        p1 = __shfl(p1, psrc, 64);
+        p2 = __shfl(p2, psrc, 64);
         float Ec[3] = {0.f, 0.f, 0.f};
-        if (grp == pg && live) {
-            const size_t c = a.cells[s];
-            const float Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
+        if (grp == g && my_live) {
             float Dc[3];
             rot(a.R, Dl, Dc);
-            float Dp[3] = {Dc[0] - P0, Dc[1] - P1, Dc[2] - P2};
+            float Dp[3] = {Dc[0] - p0, Dc[1] - p1, Dc[2] - p2};
             if (a.ade) {
-                const float El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
                 float Eo[3], Pn[3];
-                rot(a.R, El, Eo);
-                ade_step(a, s, Eo, Pn);
+                rot(a.R, Eold, Eo);
+                ade_step(a, my_slot, Eo, Pn);
                 for (int e = 0; e < 3; ++e) Dp[e] -= Pn[e];
             }
             rot(a.A, Dp, Ec);
             float El[3];
             rot_t(a.R, Ec, El);
-            a.Ex[c] = El[0];
-            a.Ey[c] = El[1];
-            a.Ez[c] = El[2];
+            a.Ex[my_cell] = El[0];
+            a.Ey[my_cell] = El[1];
+            a.Ez[my_cell] = El[2];
         }
-        // every lane of column `col` needs E_cr of that slot
-        const int src = pg * 16 + col;
-        const float E0 = __shfl(Ec[0], src, 64), E1 = __shfl(Ec[1], src, 64), E2 = __shfl(Ec[2], src, 64);
+        // E_cr of slot 16g + col as the B operand: component k in lane col + 16k
+        const int esrc = 16 * g + col;
+        const float e0 = __shfl(Ec[0], esrc, 64), e1 = __shfl(Ec[1], esrc, 64), e2 = __shfl(Ec[2], esrc, 64);
+        const float eb = grp == 0 ? e0 : grp == 1 ? e1 : grp == 2 ? e2 : 0.f;
 #pragma unroll
-        for (int mt = 0; mt < NM; ++mt) {
+        for (int mt = 0; mt < NAX; ++mt)
+            if (mt < nma) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(AQ[mt], eb, acc[mt], 0, 0, 0);
+        // FP64 integration of the dofs this lane's accumulator holds
+        const int64_t s = tile * 64 + 16 * g + col;
+        if (s < a.nslot) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int m = 16 * mt + 4 * grp + i;
-                if (m < N3 && live) {
-                    const float *q = qop + ((size_t)(mt * 4 + i) * 64 + lane) * 3;
-                    const float af = q[0] * E0 + q[1] * E1 + q[2] * E2 - acc[mt][i];
-                    const double acc64 = (double)af;
-                    const int64_t idx = tbase + (int64_t)m * 64 + sl;
-                    const double r0 = rr[idx];
-                    const double v0 = ldnt(vv + idx);
-                    stnt(r0 + (v0 * dt + acc64 * hdt2), rr + idx);
-                    stnt(v0 + acc64 * dt, vv + idx);
+            for (int mt = 0; mt < NAX; ++mt) {
+                if (mt >= nma) break;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = 16 * mt + 4 * grp + i;
+                    if (m >= N3) continue;
+                    const double am = (double)acc[mt][i];
+                    const int64_t ri = rbase + ((int64_t)g * nk + 4 * mt + grp) * 64 + col + 16 * i;
+                    const int64_t vi = vbase + (((int64_t)g * nma + mt) * 4 + i) * 64 + lane;
+                    const double r0 = rr[ri];
+                    const double v0 = ldnt(vv + vi);
+                    stnt(r0 + (v0 * dt + am * hdt2), rr + ri);
+                    stnt(v0 + am * dt, vv + vi);
                 }
             }
         }

@@ -33,7 +33,7 @@ template <typename real>
 void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
                      hipStream_t);
 void launch_reduce_r(const double *, int64_t, unsigned long long *, hipStream_t);
-void launch_argmax_r(const double *, const uint32_t *, int64_t, int, int, int, int, unsigned long long *,
+void launch_argmax_r(const double *, const uint32_t *, int64_t, IonLayout, int, int, int, unsigned long long *,
                      hipStream_t);
 template <typename real>
 void launch_slice(const real *, const real *, const real *, const real *, int, char, int, int, int, int, int,
@@ -66,6 +66,7 @@ struct Special {
     int id = 0;
     MatPre pre{};
     std::vector<uint32_t> cells;  // x-fastest field indices, ascending
+    IonLayout lay{};              // lattice state layout (plain or MFMA operand order)
     std::vector<double> qezv, qezm, dmm;
     uint32_t *d_cells = nullptr;
     void *d_P = nullptr, *d_Po = nullptr;
@@ -113,7 +114,7 @@ struct phpfdtd_ctx {
     std::vector<Special> specials;
     int special_of[kMaxMat];
     int64_t lossy_interior = 0;  // interior cells of materials with eps_im loss
-    int yee_rows = 2, yee_kchunk = 8;  // Yee tile: y rows per wavefront, z planes per chunk
+    int yee_rows = 1, yee_kchunk = 16;  // Yee tile: y rows per wavefront, z planes per chunk
     // sources
     int nsrc = 0;
     int src_i[kMaxSrc] = {}, src_j[kMaxSrc] = {}, src_k[kMaxSrc] = {}, src_c[kMaxSrc] = {};
@@ -124,7 +125,7 @@ struct phpfdtd_ctx {
     double *d_ring = nullptr;
     int64_t ring_rows = 0, mon_read = 0;
     bool mon_on = false;
-    int64_t mon_c = -1, mon_idx = -1;
+    int64_t mon_c = -1, mon_ridx = -1, mon_vidx = -1;
     const double *mon_r = nullptr, *mon_v = nullptr;
     unsigned long long *d_diag = nullptr;
     float *d_slice = nullptr;
@@ -180,6 +181,7 @@ struct phpfdtd_ctx {
 
 static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nslot,
                           const std::vector<int> &natom, const std::vector<int> &ade) {
+    const bool mfma = g.precision == PHPFDTD_FP32;
     Layout l;
     const size_t rs = g.precision == PHPFDTD_FP32 ? 4 : 8;
     l.nxp = (g.nx + 31) / 32 * 32;
@@ -197,8 +199,11 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
         if (!n) continue;
         l.slots += n * 4;
         if (ade[m]) l.slots += 2 * 3 * n * rs;
-        if (natom[m]) l.slots += 2 * 3 * (size_t)natom[m] * ((n + 63) / 64 * 64) * 8 + 9 * natom[m] * natom[m] * 8 +
-                                  18 * natom[m] * 8 + 7 * 24 * 64 * 4 + 7 * 4 * 64 * 3 * 4;
+        if (natom[m]) {
+            const IonLayout il = ion_layout(natom[m], mfma);
+            l.slots += (n + 63) / 64 * (size_t)(il.tile_r() + il.tile_v()) * 8 + 9 * natom[m] * natom[m] * 8 +
+                       18 * natom[m] * 8 + 7 * 24 * 64 * 4 + 7 * 64 * 4;
+        }
     }
     l.ring = (size_t)g.tsteps * 12 * 8;
     return l;
@@ -227,17 +232,17 @@ static int count_medium(const phpfdtd_grid &g, const int32_t *medium, int nmat, 
     return 0;
 }
 
-// MFMA operand tables of the precision-form lattice kernel for the
-// instantiation mfma_inst(natom): A = [DMM; 0; QeZV; 0] in lane order
-// (A[m][k] in lane m%16 + 16 (k%4) of tile (m/16, k/4)) and QeZM^T rows in the
-// accumulator layout (row m = 16 mt + 4 (l/16) + i held by lane l, element i).
+// MFMA operand tables of the precision-form lattice kernel, in lane order
+// (A[m][k] of m-tile mt, k-tile kt in lane m%16 + 16 (k%4)):
+//   aop: [-DMM; 0; QeZV; 0], the 3 QeZV rows at row roundup(3N, 4);
+//   qop: QeZM^T, column k = E component (k < 3), for the acceleration rows.
 static void mfma_tables(const Special &sp, std::vector<float> *aop, std::vector<float> *qop) {
     const int N = sp.pre.natom, N3 = 3 * N;
-    const int na = mfma_inst(N), NM = mfma_nm(na), NK = mfma_nk(na);
+    const int NM = mfma_nm(N), NK = mfma_nk(N), NA = mfma_nma(N);
     const int PR = (N3 + 3) / 4 * 4;
     auto A = [&](int m, int k) -> double {
         if (k >= N3) return 0.0;
-        if (m < N3) return sp.dmm[(size_t)m * N3 + k];
+        if (m < N3) return -sp.dmm[(size_t)m * N3 + k];
         if (m >= PR && m < PR + 3) return sp.qezv[(k / 3) * 9 + (m - PR) * 3 + (k % 3)];
         return 0.0;
     };
@@ -245,15 +250,12 @@ static void mfma_tables(const Special &sp, std::vector<float> *aop, std::vector<
     for (int mt = 0; mt < NM; ++mt)
         for (int kt = 0; kt < NK; ++kt)
             for (int l = 0; l < 64; ++l) (*aop)[((size_t)mt * NK + kt) * 64 + l] = (float)A(16 * mt + l % 16, 4 * kt + l / 16);
-    qop->assign((size_t)NM * 4 * 64 * 3, 0.f);
-    for (int mt = 0; mt < NM; ++mt)
-        for (int i = 0; i < 4; ++i)
-            for (int l = 0; l < 64; ++l) {
-                const int m = 16 * mt + 4 * (l / 16) + i;
-                if (m >= N3) continue;
-                for (int e = 0; e < 3; ++e)
-                    (*qop)[(((size_t)mt * 4 + i) * 64 + l) * 3 + e] = (float)sp.qezm[(m / 3) * 9 + e * 3 + (m % 3)];
-            }
+    qop->assign((size_t)NA * 64, 0.f);
+    for (int mt = 0; mt < NA; ++mt)
+        for (int l = 0; l < 64; ++l) {
+            const int m = 16 * mt + l % 16, k = l / 16;
+            if (m < N3 && k < 3) (*qop)[(size_t)mt * 64 + l] = (float)sp.qezm[(m / 3) * 9 + k * 3 + (m % 3)];
+        }
 }
 
 template <typename real>
@@ -345,6 +347,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         sa.Po = (real *)sp.d_Po;
         sa.natom = sp.pre.natom;
         sa.r = sp.d_r; sa.v = sp.d_v;
+        sa.lay = sp.lay;
         sa.dmm = sp.d_dmm; sa.qezv = sp.d_qezv; sa.qezm = sp.d_qezm;
         sa.aop = sp.d_aop; sa.qop = sp.d_qop;
         sa.dt = x->g.dt;
@@ -376,7 +379,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
     for (int q = 0; q < 9; ++q) se.F[q] = x->F[q];
     se.mon_r = x->mon_r;
     se.mon_v = x->mon_v;
-    se.mon_idx = x->mon_idx;
+    se.mon_ridx = x->mon_ridx;
+    se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
     Kernel ke;
     ke.name = "step_end";
@@ -466,7 +470,8 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     x->nmat = nmat;
     const char *ng = getenv("PHPFDTD_NO_GRAPH");
     x->use_graph = !(ng && ng[0] == '1');
-    // Yee tile shape (defaults measured on MI355X); overridable for tuning
+    // Yee tile shape (defaults measured on MI355X, 1024x1024x640); overridable for tuning
+    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 8 : 16;
     if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
         const int v = atoi(e);
         if (v == 1 || v == 2 || v == 4) x->yee_rows = v;
@@ -568,10 +573,10 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
             if ((rc = x->alloc(&sp.d_Po, 3 * n * x->rs))) return rc;
         }
         if (sp.pre.natom > 0) {
-            const size_t n3 = 3 * (size_t)sp.pre.natom;
-            const size_t npad = (n + 63) / 64 * 64;  // 64-slot tiles
-            if ((rc = x->alloc((void **)&sp.d_r, n3 * npad * 8))) return rc;
-            if ((rc = x->alloc((void **)&sp.d_v, n3 * npad * 8))) return rc;
+            sp.lay = ion_layout(sp.pre.natom, x->prec == PHPFDTD_FP32);
+            const size_t ntile = (n + 63) / 64;  // 64-slot tiles
+            if ((rc = x->alloc((void **)&sp.d_r, ntile * sp.lay.tile_r() * 8))) return rc;
+            if ((rc = x->alloc((void **)&sp.d_v, ntile * sp.lay.tile_v() * 8))) return rc;
             if ((rc = x->upload((void **)&sp.d_dmm, sp.dmm))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezv, sp.qezv))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezm, sp.qezm))) return rc;
@@ -653,14 +658,15 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
     const int zf = ref_compat ? mz + 1 : mz;
     x->mon_on = true;
     x->mon_c = zf < x->nz ? (int64_t)x->fidx(mx, my, zf) : -1;
-    x->mon_idx = -1;
+    x->mon_ridx = x->mon_vidx = -1;
     x->mon_r = x->mon_v = nullptr;
     const int m = x->medium[x->fidx(mx, my, mz)];
     const int sp = x->special_of[m];
     if (sp >= 0 && atom < x->pre[m].natom) {
         int64_t slot = 0;
         find_slot(x, sp, x->fidx(mx, my, mz), &slot);
-        x->mon_idx = ion_index(slot, atom * 3 + dir, 3 * x->pre[m].natom);
+        x->mon_ridx = x->specials[sp].lay.r_index(slot, atom * 3 + dir);
+        x->mon_vidx = x->specials[sp].lay.v_index(slot, atom * 3 + dir);
         x->mon_r = x->specials[sp].d_r;
         x->mon_v = x->specials[sp].d_v;
     }
@@ -708,10 +714,10 @@ int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
                                x->d_diag, x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
-            launch_reduce_r(sp.d_r, (int64_t)(sp.cells.size() + 63) / 64 * 64 * 3 * sp.pre.natom, x->d_diag, x->stream);
+            launch_reduce_r(sp.d_r, (int64_t)(sp.cells.size() + 63) / 64 * sp.lay.tile_r(), x->d_diag, x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
-            launch_argmax_r(sp.d_r, sp.d_cells, (int64_t)sp.cells.size(), 3 * sp.pre.natom, x->ny, x->nz, x->nxp,
+            launch_argmax_r(sp.d_r, sp.d_cells, (int64_t)sp.cells.size(), sp.lay, x->ny, x->nz, x->nxp,
                             x->d_diag, x->stream);
     unsigned long long res[4];
     HIPCHK(hipMemcpyAsync(res, x->d_diag, sizeof(res), hipMemcpyDeviceToHost, x->stream));
@@ -825,10 +831,11 @@ int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double 
     const size_t n = s.cells.size();
     const int n3 = 3 * x->pre[m].natom;
     HIPCHK(hipStreamSynchronize(x->stream));
-    const int64_t base = ion_index(slot, 0, n3);  // dof q at base + 64 q
     (void)n;
-    if (r) HIPCHK(hipMemcpy2D(r, 8, s.d_r + base, 64 * 8, 8, n3, hipMemcpyDeviceToHost));
-    if (v) HIPCHK(hipMemcpy2D(v, 8, s.d_v + base, 64 * 8, 8, n3, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n3; ++q) {
+        if (r) HIPCHK(hipMemcpy(r + q, s.d_r + s.lay.r_index(slot, q), 8, hipMemcpyDeviceToHost));
+        if (v) HIPCHK(hipMemcpy(v + q, s.d_v + s.lay.v_index(slot, q), 8, hipMemcpyDeviceToHost));
+    }
     return x->pre[m].natom;
 }
 
@@ -909,14 +916,15 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     }
     HIPCHK(hipStreamSynchronize(x->stream));
     const bool tiled = kind < 2;  // lattice r, v in 64-slot tiles; ADE P, Po as [3][nslot]
-    const size_t nelem = tiled ? (n + 63) / 64 * 64 * width : n * width;
+    const size_t nelem = tiled ? (n + 63) / 64 * (size_t)(kind == 0 ? s.lay.tile_r() : s.lay.tile_v()) : n * width;
     std::vector<unsigned char> h(nelem * es);
     const std::vector<int64_t> ord = file_order(x, s);
     if (!write || tiled) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < n; ++q) {
         const int64_t slot = ord[q];
         for (int d = 0; d < width; ++d) {
-            const size_t hi = tiled ? (size_t)ion_index(slot, d, width) : (size_t)d * n + slot;
+            const size_t hi = tiled ? (size_t)(kind == 0 ? s.lay.r_index(slot, d) : s.lay.v_index(slot, d))
+                                    : (size_t)d * n + slot;
             double &u = buf[q * width + d];
             if (es == 8) {
                 if (write) ((double *)h.data())[hi] = u;

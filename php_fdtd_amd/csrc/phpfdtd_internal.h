@@ -75,6 +75,74 @@ struct YeeArgs {
     const int64_t *step;
 };
 
+// MFMA tiling of the lattice contraction for N atoms: k-tiles of 4 dofs over
+// the 3N dofs, m-tiles of 16 rows over [DMM; 0; QeZV] (the 3 QeZV rows sit at
+// row roundup(3N, 4)), and m-tiles over the 3N acceleration rows alone.
+__host__ __device__ inline int mfma_nk(int na) { return (3 * na + 3) / 4; }
+__host__ __device__ inline int mfma_nm(int na) { return ((3 * na + 3) / 4 * 4 + 3 + 15) / 16; }
+__host__ __device__ inline int mfma_nma(int na) { return (3 * na + 15) / 16; }
+
+// natom of the MFMA kernel instantiation (register arrays sized for it; loops
+// run to the actual tile counts of the material).
+inline int mfma_inst(int natom) {
+    switch (natom) {
+        case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return natom;
+        default: return kMaxAtom;
+    }
+}
+
+// Lattice state layout of one material (r and v, FP64, tiles of 64 slots).
+//  plain form (mfma = 0): r[tile][dof][64] and v likewise - a thread owns a
+//    slot and reads 64 consecutive slots of one dof per wavefront access.
+//  precision form (mfma = 1): r in MFMA B-operand order
+//    r[tile][g][kt][lane], lane = n + 16 (dof % 4), dof = 4 kt + lane / 16,
+//    slot = 16 g + n; v in MFMA accumulator order
+//    v[tile][g][mt][i][lane], dof = 16 mt + 4 (lane / 16) + i, slot = 16 g + lane % 16.
+//  Every wavefront access of the lattice kernels is then 512 contiguous bytes.
+struct IonLayout {
+    int mfma, n3, nk, nma;
+    __host__ __device__ int64_t tile_r() const { return mfma ? 256LL * nk : 64LL * n3; }
+    __host__ __device__ int64_t tile_v() const { return mfma ? 1024LL * nma : 64LL * n3; }
+    __host__ __device__ int64_t r_index(int64_t s, int q) const {
+        const int64_t t = s >> 6;
+        const int sl = (int)(s & 63);
+        if (!mfma) return t * tile_r() + (int64_t)q * 64 + sl;
+        const int g = sl >> 4, n = sl & 15;
+        return t * tile_r() + ((int64_t)g * nk + (q >> 2)) * 64 + n + 16 * (q & 3);
+    }
+    __host__ __device__ int64_t v_index(int64_t s, int q) const {
+        const int64_t t = s >> 6;
+        const int sl = (int)(s & 63);
+        if (!mfma) return t * tile_v() + (int64_t)q * 64 + sl;
+        const int g = sl >> 4, n = sl & 15;
+        const int mt = q >> 4, lg = (q & 15) >> 2, i = q & 3;
+        return t * tile_v() + (((int64_t)g * nma + mt) * 4 + i) * 64 + n + 16 * lg;
+    }
+    // slot and dof of r element idx; false for padding
+    __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
+        const int64_t t = idx / tile_r(), rem = idx % tile_r();
+        if (!mfma) {
+            *q = (int)(rem / 64);
+            *s = t * 64 + rem % 64;
+        } else {
+            const int gk = (int)(rem / 64), lane = (int)(rem % 64);
+            const int g = gk / nk, kt = gk % nk;
+            *q = 4 * kt + lane / 16;
+            *s = t * 64 + 16 * g + lane % 16;
+        }
+        return *q < n3 && *s < nslot;
+    }
+};
+
+inline IonLayout ion_layout(int natom, bool mfma) {
+    IonLayout l;
+    l.mfma = mfma ? 1 : 0;
+    l.n3 = 3 * natom;
+    l.nk = mfma_nk(natom);
+    l.nma = mfma_nma(natom);
+    return l;
+}
+
 // Lattice / ADE slot kernels: one material per launch.
 template <typename real>
 struct SlotArgs {
@@ -91,31 +159,15 @@ struct SlotArgs {
     real *P, *Po;  // [3][nslot]
     // lattice (fdtd.c:3505-3528)
     int natom;
-    double *r, *v;           // [tile][3N][64] (64-slot tiles), FP64 state in both forms
+    double *r, *v;           // FP64 state in both forms, layout `lay`
+    IonLayout lay;
     const double *qezv;      // [N][3][3]
     const double *qezm;      // [N][3][3]
     const double *dmm;       // [3N][3N]
-    const float *aop;        // precision form: [DMM; QeZV] MFMA A operands in lane order
-    const float *qop;        // precision form: QeZM^T rows in the MFMA D layout
+    const float *aop;        // precision form: [-DMM; 0; QeZV] MFMA A operands in lane order
+    const float *qop;        // precision form: QeZM^T MFMA A operands in lane order
     double dt, half_dt2;
 };
-
-// Offset of dof q of slot s in a lattice state array (64-slot tiles).
-__host__ __device__ inline int64_t ion_index(int64_t s, int q, int n3) {
-    return (s >> 6) * (int64_t)n3 * 64 + (int64_t)q * 64 + (s & 63);
-}
-
-// natom of the MFMA kernel instantiation used for a lattice of natom atoms,
-// and its m-tile / k-tile counts (16 rows / 4 dofs per tile).  The 3 QeZV rows
-// sit at row roundup(3N, 4) below the 3N DMM rows.
-inline int mfma_inst(int natom) {
-    switch (natom) {
-        case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return natom;
-        default: return kMaxAtom;
-    }
-}
-inline int mfma_nk(int na) { return (3 * na + 3) / 4; }
-inline int mfma_nm(int na) { return ((3 * na + 3) / 4 * 4 + 3 + 15) / 16; }
 
 // Per-step epilogue: monitor row + step counter.
 struct StepEndArgs {
@@ -126,7 +178,7 @@ struct StepEndArgs {
     int64_t mon_c;     // field index sampled (z+1 in reference mode), -1 = zero halo
     const void *F[9];  // field arrays (real)
     const double *mon_r, *mon_v;  // lattice state of the monitor cell's material
-    int64_t mon_idx;   // ion_index of (slot, atom*3+dir), -1 = no R/V
+    int64_t mon_ridx, mon_vidx;   // offsets of (slot, atom*3+dir) in r and v, -1 = no R/V
     double dt;
 };
 
