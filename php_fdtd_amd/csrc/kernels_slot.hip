@@ -164,9 +164,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 //      k in lane n + 16k) and acc += QeZM^T . E_cr, one MFMA per m-tile, so
 //      acc = QeZM^T E_cr - DMM r^n = a.
 //   3. each lane integrates the dofs its accumulator holds in FP64: v from
-//      the accumulator-ordered state, r^n from an LDS copy of the FP64 values
-//      step 1 loaded (rows of 17 doubles per dof: the 4 lane groups of a read
-//      fall in different banks).
+//      the accumulator-ordered state, r^n re-read from the cache lines step 1
+//      brought in (served by the L2).
+// The groups are software-pipelined: the r^n operands of group g+1 and the v
+// of group g are requested before group g's MFMA chain, so every wavefront
+// keeps its next memory round trip in flight while it computes.
 // aop[(mt*nk + kt)*64 + l] and qop[mt*64 + l] hold the A operands in lane order.
 template <int NA>
 __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
@@ -191,8 +193,6 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
     const int pm = PR >> 4, pg = (PR & 15) >> 2;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t rbase = tile * L.tile_r(), vbase = tile * L.tile_v();
-    extern __shared__ double lds_r[];
-    double *rs = lds_r + (threadIdx.x >> 6) * (NKX * 4 * 17);  // this wavefront's [dof][17]
 
     float A[NMX][NKX], AQ[NAX];
 #pragma unroll
@@ -219,25 +219,37 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
         }
     }
     const double dt = a.dt, hdt2 = a.half_dt2;
+    // r^n operands of group 0 (cached loads: step 3 re-reads these lines)
+    double bnext[NKX];
+#pragma unroll
+    for (int kt = 0; kt < NKX; ++kt) bnext[kt] = kt < nk ? rr[rbase + (int64_t)kt * 64 + lane] : 0.0;
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
-        // lanes exchange r^n through LDS within the wavefront: keep the previous
-        // group's reads ahead of this group's writes
-        __builtin_amdgcn_wave_barrier();
+        float b[NKX];
+#pragma unroll
+        for (int kt = 0; kt < NKX; ++kt) b[kt] = (float)bnext[kt];
+        // v of this group's accumulator dofs, and r^n of the next group
+        double vpre[NAX][4];
+#pragma unroll
+        for (int mt = 0; mt < NAX; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                vpre[mt][i] = (mt < nma) ? ldnt(vv + vbase + (((int64_t)g * nma + mt) * 4 + i) * 64 + lane) : 0.0;
+        if (g < 3) {
+#pragma unroll
+            for (int kt = 0; kt < NKX; ++kt)
+                bnext[kt] = kt < nk ? rr[rbase + ((int64_t)(g + 1) * nk + kt) * 64 + lane] : 0.0;
+        }
         f32x4 acc[NMX];
 #pragma unroll
         for (int mt = 0; mt < NMX; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < NKX; ++kt) {
             if (kt >= nk) break;
-            const double r64 = ldnt(rr + rbase + ((int64_t)g * nk + kt) * 64 + lane);
-            rs[(4 * kt + grp) * 17 + col] = r64;
-            const float b = (float)r64;
 #pragma unroll
             for (int mt = 0; mt < NMX; ++mt)
-                if (mt < nm) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b, acc[mt], 0, 0, 0);
+                if (mt < nm) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b[kt], acc[mt], 0, 0, 0);
         }
-        __builtin_amdgcn_wave_barrier();  // LDS writes of r^n before the reads of step 3
         // P_ion of slot 16g + col to lane 16g + col
         float p0 = 0.f, p1 = 0.f, p2 = 0.f;
 #pragma unroll
@@ -283,10 +295,11 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
                     const int m = 16 * mt + 4 * grp + i;
                     if (m >= N3) continue;
                     const double am = (double)acc[mt][i];
-                    const int64_t ri = rbase + ((int64_t)g * nk + 4 * mt + grp) * 64 + col + 16 * i;  // B-order slot of (m, col)
+                    // B-order slot of (dof m, slot col) and its accumulator-order v
+                    const int64_t ri = rbase + ((int64_t)g * nk + 4 * mt + grp) * 64 + col + 16 * i;
                     const int64_t vi = vbase + (((int64_t)g * nma + mt) * 4 + i) * 64 + lane;
-                    const double r0 = rs[m * 17 + col];
-                    const double v0 = ldnt(vv + vi);
+                    const double r0 = rr[ri];
+                    const double v0 = vpre[mt][i];
                     stnt(r0 + (v0 * dt + am * hdt2), rr + ri);
                     stnt(v0 + am * dt, vv + vi);
                 }
@@ -337,8 +350,7 @@ void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
     switch (mfma_inst(a.natom)) {
 #define PHPFDTD_MFMA_CASE(NN)                                                                                \
     case NN:                                                                                                 \
-        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3(nb), dim3(256), (size_t)4 * mfma_nk(NN) * 4 * 17 * 8, st, a, \
-                           a.aop, a.qop, a.r, a.v);                                                          \
+        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3(nb), dim3(256), 0, st, a, a.aop, a.qop, a.r, a.v);     \
         return;
         PHPFDTD_MFMA_CASE(1)
         PHPFDTD_MFMA_CASE(2)

@@ -38,6 +38,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks, int nblocks_pad) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Wavefront-uniform table reads (per-plane z coefficients, per-row y
+// coefficients, source values) go through the scalar cache: the constant
+// address space is read-only for the kernel's lifetime, so these become
+// s_load and wait on their own counter instead of queueing behind the field
+// streams in the in-order vector-memory counter.
+template <typename T>
+__device__ __forceinline__ T sld(const T *p, int64_t idx) {
+    typedef const __attribute__((address_space(4))) T *cptr;
+    return ((cptr)p)[idx];
+}
+
 template <typename real>
 struct Vec;
 template <>
@@ -101,13 +112,15 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
     int px[VN];
 #pragma unroll
     for (int e = 0; e < VN; ++e) px[e] = x0 + e < a.nx - 1 ? a.ax.slot[xr + e] : -1;
-    real khy[R];
+    real khy[R], byr[R], chr_[R];
     int py[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = min(j0 + r, a.ny - 1);
-        khy[r] = a.ay.kh[j];
-        py[r] = uniform(a.ay.slot[j]);
+        khy[r] = sld(a.ay.kh, j);
+        py[r] = sld(a.ay.slot, j);
+        byr[r] = sld(a.ay.b, j);
+        chr_[r] = sld(a.ay.ch, j);
     }
     bool anypx = false;
 #pragma unroll
@@ -134,10 +147,9 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (r < nr) eyn[r] = top ? V(0) : *reinterpret_cast<const V *>(a.Ey + c0 + plane + r * nxp);
-        const real khz = a.az.kh[k];
-        const int pz = uniform(a.az.slot[k]);
-        real bzv = 0, chz = 0;
-        if (pz >= 0) { bzv = a.az.b[k]; chz = a.az.ch[k]; }
+        const real khz = sld(a.az.kh, k);
+        const int pz = sld(a.az.slot, k);
+        const real bzv = sld(a.az.b, k), chz = sld(a.az.ch, k);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (r >= nr) continue;
@@ -182,7 +194,7 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
             if (py[r] >= 0) {
                 V *syxp = reinterpret_cast<V *>(a.pHyx + ((size_t)k * L2 + py[r]) * nxp + xr);
                 V *syzp = reinterpret_cast<V *>(a.pHyz + ((size_t)k * L2 + py[r]) * nxp + xr);
-                const real byv = a.ay.b[j], chy = a.ay.ch[j];
+                const real byv = byr[r], chy = chr_[r];
                 const V syx = *syxp, syz = *syzp;
                 nhx += syx;
                 nhz += syz;
@@ -272,19 +284,21 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
 #pragma unroll
     for (int e = 0; e < VN; ++e) anypx |= px[e] >= 0 && inx[e];
     bool iny[R];
-    real kdy[R];
+    real kdy[R], byr[R], cdr[R];
     int py[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = j0 + r;
         iny[r] = j >= 1 && j <= a.ny - 2 && r < nr;
         const int jc = min(j, a.ny - 1);
-        kdy[r] = a.ay.kd[jc];
-        py[r] = uniform(a.ay.slot[jc]);
+        kdy[r] = sld(a.ay.kd, jc);
+        py[r] = sld(a.ay.slot, jc);
+        byr[r] = sld(a.ay.b, jc);
+        cdr[r] = sld(a.ay.cd, jc);
     }
     const bool halo = j0 >= 1;
 
-    const int64_t st = a.nsrc > 0 ? *a.step : 0;
+    const int64_t st = a.nsrc > 0 ? sld(a.step, 0) : 0;
 
     size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + xr;
     V hxm[R], hym[R];
@@ -312,10 +326,9 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
                 hz[r + 1] = *reinterpret_cast<const V *>(a.Hz + c0 + r * nxp);
                 hy[r] = *reinterpret_cast<const V *>(a.Hy + c0 + r * nxp);
             }
-        const real kdz = a.az.kd[k];
-        const int pz = uniform(a.az.slot[k]);
-        real bzv = 0, cdz = 0;
-        if (pz >= 0) { bzv = a.az.b[k]; cdz = a.az.cd[k]; }
+        const real kdz = sld(a.az.kd, k);
+        const int pz = sld(a.az.slot, k);
+        const real bzv = sld(a.az.b, k), cdz = sld(a.az.cd, k);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (r >= nr) continue;
@@ -370,7 +383,7 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
                 if (py[r] >= 0) {
                     V *syxp = reinterpret_cast<V *>(a.pDyx + ((size_t)k * L2 + py[r]) * nxp + xr);
                     V *syzp = reinterpret_cast<V *>(a.pDyz + ((size_t)k * L2 + py[r]) * nxp + xr);
-                    const real byv = a.ay.b[j], cdy = a.ay.cd[j];
+                    const real byv = byr[r], cdy = cdr[r];
                     const V syx = *syxp, syz = *syzp;
                     ndx += syx;
                     ndz += syz;
@@ -418,7 +431,7 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
                 if (a.src_k[s] == k && a.src_j[s] == j) {
                     const int e = a.src_i[s] - x0;
                     if (e >= 0 && e < VN) {
-                        const real v = a.src_val[st * a.nsrc + s];
+                        const real v = sld(a.src_val, st * a.nsrc + s);
 #pragma unroll
                         for (int q = 0; q < VN; ++q)
                             if (q == e) {
