@@ -154,21 +154,21 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // layouts of v_mfma_f32_16x16x4f32 (lane l, gfx950):
 //   A[m][k] in lane m + 16k,  B[k][n] in lane n + 16k,
 //   D[m][n] in lane n + 16 (m / 4), element m % 4.
-// Per 16-slot group g of the tile:
-//   1. acc = [-DMM; 0; QeZV] . r^n over NK k-tiles (B = r^n rounded to FP32,
-//      read straight from the B-ordered state); the QeZV rows at
-//      PR = roundup(3N, 4) give P_ion in elements 0..2 of lane group
-//      (PR % 16) / 4 of m-tile PR / 16.
+// r and v are stored in B-operand order (IonLayout, mfma = 1), so lane l of
+// k-tile kt of group g owns dof 4 kt + l / 16 of slot 16 g + l % 16 and every
+// global access of the kernel is one 512-byte row.  Per 16-slot group g:
+//   1. acc = [-DMM; 0; QeZV] . r^n over the k-tiles (B = r^n rounded to
+//      FP32); the QeZV rows at PR = roundup(3N, 4) give P_ion in elements
+//      0..2 of lane group (PR % 16) / 4 of m-tile PR / 16.
 //   2. lane 16g + n (which prefetched D of slot 16g + n) forms E_cr of its
-//      slot from P_ion; E_cr is spread as a 4th-operand k-tile (E component
-//      k in lane n + 16k) and acc += QeZM^T . E_cr, one MFMA per m-tile, so
-//      acc = QeZM^T E_cr - DMM r^n = a.
-//   3. each lane integrates the dofs its accumulator holds in FP64: v from
-//      the accumulator-ordered state, r^n re-read from the cache lines step 1
-//      brought in (served by the L2).
-// The groups are software-pipelined: the r^n operands of group g+1 and the v
-// of group g are requested before group g's MFMA chain, so every wavefront
-// keeps its next memory round trip in flight while it computes.
+//      slot; E_cr is spread as one more k-tile (E component k in lane
+//      n + 16k) and acc += QeZM^T . E_cr, one MFMA per m-tile, so
+//      acc = QeZM^T E_cr - DMM r^n = a (accumulator order).
+//   3. a is transposed to B order through a 17-float-pitch LDS tile of the
+//      wavefront, and each lane integrates its own dofs in FP64 from the r^n
+//      it still holds and the v it prefetched.
+// The groups are software-pipelined: r^n of group g+1 and v of group g are
+// requested before group g's MFMA chain.
 // aop[(mt*nk + kt)*64 + l] and qop[mt*64 + l] hold the A operands in lane order.
 template <int NA>
 __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
@@ -192,7 +192,9 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
     const int PR = (N3 + 3) & ~3;
     const int pm = PR >> 4, pg = (PR & 15) >> 2;
     const int grp = lane >> 4, col = lane & 15;
-    const int64_t rbase = tile * L.tile_r(), vbase = tile * L.tile_v();
+    const int64_t base = tile * L.tile_r();
+    extern __shared__ float lds_a[];
+    float *at = lds_a + (threadIdx.x >> 6) * (NKX * 4 * 17);  // this wavefront's a[dof][17]
 
     float A[NMX][NKX], AQ[NAX];
 #pragma unroll
@@ -219,26 +221,21 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
         }
     }
     const double dt = a.dt, hdt2 = a.half_dt2;
-    // r^n operands of group 0 (cached loads: step 3 re-reads these lines)
-    double bnext[NKX];
+    double rnext[NKX];
 #pragma unroll
-    for (int kt = 0; kt < NKX; ++kt) bnext[kt] = kt < nk ? rr[rbase + (int64_t)kt * 64 + lane] : 0.0;
+    for (int kt = 0; kt < NKX; ++kt) rnext[kt] = kt < nk ? ldnt(rr + base + (int64_t)kt * 64 + lane) : 0.0;
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
-        float b[NKX];
+        double rcur[NKX], vcur[NKX];
 #pragma unroll
-        for (int kt = 0; kt < NKX; ++kt) b[kt] = (float)bnext[kt];
-        // v of this group's accumulator dofs, and r^n of the next group
-        double vpre[NAX][4];
-#pragma unroll
-        for (int mt = 0; mt < NAX; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                vpre[mt][i] = (mt < nma) ? ldnt(vv + vbase + (((int64_t)g * nma + mt) * 4 + i) * 64 + lane) : 0.0;
+        for (int kt = 0; kt < NKX; ++kt) {
+            rcur[kt] = rnext[kt];
+            vcur[kt] = kt < nk ? ldnt(vv + base + ((int64_t)g * nk + kt) * 64 + lane) : 0.0;
+        }
         if (g < 3) {
 #pragma unroll
             for (int kt = 0; kt < NKX; ++kt)
-                bnext[kt] = kt < nk ? rr[rbase + ((int64_t)(g + 1) * nk + kt) * 64 + lane] : 0.0;
+                rnext[kt] = kt < nk ? ldnt(rr + base + ((int64_t)(g + 1) * nk + kt) * 64 + lane) : 0.0;
         }
         f32x4 acc[NMX];
 #pragma unroll
@@ -246,9 +243,10 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
 #pragma unroll
         for (int kt = 0; kt < NKX; ++kt) {
             if (kt >= nk) break;
+            const float b = (float)rcur[kt];
 #pragma unroll
             for (int mt = 0; mt < NMX; ++mt)
-                if (mt < nm) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b[kt], acc[mt], 0, 0, 0);
+                if (mt < nm) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b, acc[mt], 0, 0, 0);
         }
         // P_ion of slot 16g + col to lane 16g + col
         float p0 = 0.f, p1 = 0.f, p2 = 0.f;
@@ -284,26 +282,28 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
 #pragma unroll
         for (int mt = 0; mt < NAX; ++mt)
             if (mt < nma) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(AQ[mt], eb, acc[mt], 0, 0, 0);
-        // FP64 integration of the dofs this lane's accumulator holds
-        const int64_t s = tile * 64 + 16 * g + col;
-        if (s < a.nslot) {
+        // a from accumulator order to B order through LDS (same wavefront only)
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int mt = 0; mt < NAX; ++mt) {
-                if (mt >= nma) break;
+        for (int mt = 0; mt < NAX; ++mt) {
+            if (mt >= nma) break;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int m = 16 * mt + 4 * grp + i;
-                    if (m >= N3) continue;
-                    const double am = (double)acc[mt][i];
-                    // B-order slot of (dof m, slot col) and its accumulator-order v
-                    const int64_t ri = rbase + ((int64_t)g * nk + 4 * mt + grp) * 64 + col + 16 * i;
-                    const int64_t vi = vbase + (((int64_t)g * nma + mt) * 4 + i) * 64 + lane;
-                    const double r0 = rr[ri];
-                    const double v0 = vpre[mt][i];
-                    stnt(r0 + (v0 * dt + am * hdt2), rr + ri);
-                    stnt(v0 + am * dt, vv + vi);
-                }
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * mt + 4 * grp + i;
+                if (m < N3) at[m * 17 + col] = acc[mt][i];
             }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const bool live = tile * 64 + 16 * g + col < a.nslot;
+#pragma unroll
+        for (int kt = 0; kt < NKX; ++kt) {
+            if (kt >= nk) break;
+            const int dof = 4 * kt + grp;
+            if (dof >= N3 || !live) continue;
+            const double am = (double)at[dof * 17 + col];
+            const int64_t idx = base + ((int64_t)g * nk + kt) * 64 + lane;
+            stnt(rcur[kt] + (vcur[kt] * dt + am * hdt2), rr + idx);
+            stnt(vcur[kt] + am * dt, vv + idx);
         }
     }
 }
@@ -350,7 +350,8 @@ void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
     switch (mfma_inst(a.natom)) {
 #define PHPFDTD_MFMA_CASE(NN)                                                                                \
     case NN:                                                                                                 \
-        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3(nb), dim3(256), 0, st, a, a.aop, a.qop, a.r, a.v);     \
+        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3(nb), dim3(256), (size_t)4 * mfma_nk(NN) * 4 * 17 * 4, st, a, \
+                           a.aop, a.qop, a.r, a.v);                                                          \
         return;
         PHPFDTD_MFMA_CASE(1)
         PHPFDTD_MFMA_CASE(2)

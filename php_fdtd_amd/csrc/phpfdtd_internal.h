@@ -94,15 +94,14 @@ inline int mfma_inst(int natom) {
 // Lattice state layout of one material (r and v, FP64, tiles of 64 slots).
 //  plain form (mfma = 0): r[tile][dof][64] and v likewise - a thread owns a
 //    slot and reads 64 consecutive slots of one dof per wavefront access.
-//  precision form (mfma = 1): r in MFMA B-operand order
-//    r[tile][g][kt][lane], lane = n + 16 (dof % 4), dof = 4 kt + lane / 16,
-//    slot = 16 g + n; v in MFMA accumulator order
-//    v[tile][g][mt][i][lane], dof = 16 mt + 4 (lane / 16) + i, slot = 16 g + lane % 16.
+//  precision form (mfma = 1): r and v in MFMA B-operand order
+//    [tile][g][kt][lane], lane = n + 16 (dof % 4), dof = 4 kt + lane / 16,
+//    slot = 16 g + n.
 //  Every wavefront access of the lattice kernels is then 512 contiguous bytes.
 struct IonLayout {
     int mfma, n3, nk, nma;
     __host__ __device__ int64_t tile_r() const { return mfma ? 256LL * nk : 64LL * n3; }
-    __host__ __device__ int64_t tile_v() const { return mfma ? 1024LL * nma : 64LL * n3; }
+    __host__ __device__ int64_t tile_v() const { return tile_r(); }
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
         const int64_t t = s >> 6;
         const int sl = (int)(s & 63);
@@ -110,14 +109,7 @@ struct IonLayout {
         const int g = sl >> 4, n = sl & 15;
         return t * tile_r() + ((int64_t)g * nk + (q >> 2)) * 64 + n + 16 * (q & 3);
     }
-    __host__ __device__ int64_t v_index(int64_t s, int q) const {
-        const int64_t t = s >> 6;
-        const int sl = (int)(s & 63);
-        if (!mfma) return t * tile_v() + (int64_t)q * 64 + sl;
-        const int g = sl >> 4, n = sl & 15;
-        const int mt = q >> 4, lg = (q & 15) >> 2, i = q & 3;
-        return t * tile_v() + (((int64_t)g * nma + mt) * 4 + i) * 64 + n + 16 * lg;
-    }
+    __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
     // slot and dof of r element idx; false for padding
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
         const int64_t t = idx / tile_r(), rem = idx % tile_r();
