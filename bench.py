@@ -58,6 +58,70 @@ def run_form(spec, med, precision, warmup, steps, prof_steps):
     return dict(ms=ms, wall_ms=wall * 1e3 / steps, prof=prof, rows=rows, max_e=d.max_e, max_r=d.max_r)
 
 
+def central(a):
+    """Central 25%-75% region of a plane, strict bounds (frequency_analysis.c:300-303)."""
+    n0, n1 = a.shape
+    i = np.arange(n0)[:, None]
+    j = np.arange(n1)[None, :]
+    return a[(i > n0 * 0.25) & (i < n0 * 0.75) & (j > n1 * 0.25) & (j < n1 * 0.75)]
+
+
+def accuracy(name, interval=10):
+    """FP32 precision form vs FP64 plain form over a full-length run: monitor
+    trace relative L2 (fields; lattice R, V) and the near-field single-bin DFT
+    at the source frequency of the z plane one cell above the slab, central
+    25-75% region (frequency_analysis.c semantics)."""
+    from php_fdtd_amd import _abi, workloads
+    from php_fdtd_amd.sim import Simulation
+
+    spec, med = workloads.CONFIGS[name]()
+    g = spec["grid"]
+    T = g["tsteps"]
+    N = T // interval
+    zs = np.nonzero((med != 0).any(axis=(0, 1)))[0]
+    zplane = int(zs.max()) + 1
+    f = spec["sources"][-1]["frequency"]
+    mon = spec["monitor"]
+    res = {}
+    for prec in (_abi.FP32, _abi.FP64):
+        sim = Simulation(spec, med, prec)
+        sim.set_monitor(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"], ref_compat=True)
+        k = sim.set_dft("Ez", "z", zplane, interval, N, f, ref_compat=False)
+        t0 = time.perf_counter()
+        sim.run(T)
+        sim.sync(diag=False)
+        wall = time.perf_counter() - t0
+        rows = sim.monitor(T)
+        x, ns = sim.dft()
+        d = sim.sync()
+        sim.close()
+        res[prec] = dict(rows=rows, x=x, k=k, wall=wall, max_r=d.max_r)
+        log("  accuracy %s %s: %d steps in %.2f s" % (name, "fp32" if prec else "fp64", T, wall))
+    a, b = res[_abi.FP32], res[_abi.FP64]
+    x32, x64 = central(a["x"]), central(b["x"])
+    amp32, amp64 = np.abs(x32), np.abs(x64)
+    ph = np.angle(x32 * np.conj(x64))
+    strong = amp64 >= 0.1 * amp64.max()
+    fs = 1.0 / (interval * g["dt"])
+
+    def rl2(u, v):
+        n = np.linalg.norm(v)
+        return float(np.linalg.norm(u - v) / n) if n else None
+
+    return {
+        "workload": "%s full length: %dx%dx%d cells, %d steps, FP32 form vs FP64 plain form" % (
+            name, g["nx"], g["ny"], g["nz"], T),
+        "monitor_rel_l2": rl2(a["rows"][:, 1:10], b["rows"][:, 1:10]),
+        "monitor_rv_rel_l2": rl2(a["rows"][:, 10:12], b["rows"][:, 10:12]),
+        "nearfield": {"component": "Ez", "plane": "z=%d" % zplane, "samples": N, "interval": interval,
+                      "bin": b["k"], "bin_hz": b["k"] * fs / N, "source_hz": f,
+                      "amp_rel_l2": rl2(amp32, amp64),
+                      "phase_rms_rad": float(np.sqrt(np.sum(amp64 ** 2 * ph ** 2) / np.sum(amp64 ** 2))),
+                      "phase_max_rad_strong": float(np.abs(ph[strong]).max())},
+        "solve_s": {"fp32": round(a["wall"], 3), "fp64": round(b["wall"], 3)},
+    }
+
+
 def cpu_baseline(budget_s=25.0):
     """Reference fdtd.c (oracle/_ref/fdtd.x, -Ofast -flto -funroll-loops) under
     mpirun on the host cores, on the same workload scaled down 4x per axis
@@ -121,6 +185,7 @@ def main():
     ap.add_argument("--config", default="cfg3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64")
+    ap.add_argument("--accuracy", default="cfg1", help="config for the FP32-vs-FP64 accuracy run ('' = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the domain is resident on a single MI355X and no "
@@ -187,6 +252,8 @@ def main():
             num = np.linalg.norm(a[:n, 1:10] - b[:n, 1:10])
             den = np.linalg.norm(b[:n, 1:10])
             out["fp32_monitor_rel_l2_vs_fp64"] = float(num / den) if den else None
+    if args.accuracy:
+        out["accuracy"] = accuracy(args.accuracy)
     if not args.no_cpu:
         try:
             t0 = time.time()

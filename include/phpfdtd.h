@@ -17,6 +17,8 @@
  *                                                             phpfdtd_read_point,
  *                                                             phpfdtd_read_ions
  *   fdtd.c:555-625, 2478-2528, 962-989  host precompute    -> phpfdtd_create
+ *   frequency_analysis.c:362-390, 548-569  slice FFT bin    -> phpfdtd_set_dft,
+ *                                                             phpfdtd_read_dft
  *
  * Conventions
  *   - Every function returns 0 or a negative PHPFDTD_E* code, and never calls
@@ -182,6 +184,29 @@ int phpfdtd_read_point(phpfdtd_ctx *ctx, int x, int y, int z, double *fields9);
 /* r and v (3*natom each, m and m/s, [atom][dir]) of the ion cell (x,y,z).
  * Returns the number of atoms at that cell (0 if none). */
 int phpfdtd_read_ions(phpfdtd_ctx *ctx, int x, int y, int z, double *r, double *v);
+
+/* Near-field single-bin DFT of one plane, accumulated on the device in FP64
+ * while the simulation runs: the quantity frequency_analysis.c computes from
+ * the slice files (frequency_analysis.c:362-390 bin choice, 548-569 FFT),
+ * without storing the slice time series.
+ *   Samples: x_n = the plane after the update of step t0 + n*interval,
+ *            n = 0 .. nsamples-1, where t0 = steps completed at this call
+ *            (0 before the first phpfdtd_run, as the reference writes slices
+ *            at t % output_interval == 0, fdtd.c:3670).
+ *   Bin:     k = first argmin over i < nsamples/2 of |i*fs/N - f_hz| with
+ *            fs = 1/(interval*dt), N = nsamples.
+ *   Result:  X = sum_n x_n exp(-2 pi i k n / N) / N  (FFTW forward sign).
+ * comp, axis, pos and ref_compat as in phpfdtd_read_slice; ref_compat=1 also
+ * samples the float32 value a slice file holds.  Replaces a previous DFT and
+ * zeroes its accumulator.  Adds 16 bytes per plane cell and per sample to the
+ * context's device memory.  *bin (may be NULL) receives k. */
+int phpfdtd_set_dft(phpfdtd_ctx *ctx, int comp, char axis, int pos, int ref_compat, int interval,
+                    int nsamples, double f_hz, int *bin);
+
+/* Real and imaginary parts of X in the slice layout of phpfdtd_read_slice
+ * (re or im may be NULL); *n_samples (may be NULL) receives the samples
+ * accumulated so far. */
+int phpfdtd_read_dft(phpfdtd_ctx *ctx, double *re, double *im, int *n_samples);
 
 /* Whole component in file order, FP64 (tests, checkpoints). */
 int phpfdtd_read_field(phpfdtd_ctx *ctx, int comp, double *out);

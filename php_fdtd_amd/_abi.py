@@ -177,6 +177,9 @@ def load_library(path=None):
     lib.phpfdtd_read_point.argtypes = [c_ctx, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
     lib.phpfdtd_read_ions.argtypes = [c_ctx, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_double),
                                       P(ctypes.c_double)]
+    lib.phpfdtd_set_dft.argtypes = [c_ctx, ctypes.c_int, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_double, P(ctypes.c_int)]
+    lib.phpfdtd_read_dft.argtypes = [c_ctx, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
     lib.phpfdtd_read_field.argtypes = [c_ctx, ctypes.c_int, P(ctypes.c_double)]
     lib.phpfdtd_write_field.argtypes = [c_ctx, ctypes.c_int, P(ctypes.c_double)]
     lib.phpfdtd_count_cells.argtypes = [c_ctx, ctypes.c_int, P(ctypes.c_int64)]
@@ -201,6 +204,7 @@ def load_library(path=None):
 EXPORTED = [
     "phpfdtd_create", "phpfdtd_set_sources", "phpfdtd_set_monitor", "phpfdtd_run", "phpfdtd_sync",
     "phpfdtd_read_slice", "phpfdtd_read_monitor", "phpfdtd_read_point", "phpfdtd_read_ions",
+    "phpfdtd_set_dft", "phpfdtd_read_dft",
     "phpfdtd_read_field", "phpfdtd_write_field", "phpfdtd_count_cells", "phpfdtd_read_state",
     "phpfdtd_write_state", "phpfdtd_footprint_bytes", "phpfdtd_profile", "phpfdtd_time_steps",
     "phpfdtd_step", "phpfdtd_last_error", "phpfdtd_abi_version", "phpfdtd_destroy",

@@ -101,35 +101,72 @@ __global__ __launch_bounds__(256) void argmax_r(const double *r, const uint32_t 
     if ((threadIdx.x & 63) == 0) atomicMin(out + 3, best);
 }
 
-// One plane as float32 in the reference's slice layout (fdtd.c:999-1400).
-// comp < 0 selects |E| computed in FP64 from Ex, Ey, Ez (fdtd.c:3693-3703).
+// Field index of element q of a plane in the reference's slice layout
+// (fdtd.c:999-1400): axis 'z' -> [nx][ny], 'x' -> [ny][nz], 'y' -> [nx][nz].
+// Returns -1 past the end of the plane.
+__device__ __forceinline__ int64_t plane_cell(int64_t q, char axis, int pos, int nx, int ny, int nz, int nxp) {
+    if (axis == 'z') {
+        if (q >= (int64_t)nx * ny) return -1;
+        const int i = (int)(q / ny), j = (int)(q % ny);
+        return ((int64_t)pos * ny + j) * nxp + i;
+    }
+    if (axis == 'x') {
+        if (q >= (int64_t)ny * nz) return -1;
+        const int j = (int)(q / nz), k = (int)(q % nz);
+        return ((int64_t)k * ny + j) * nxp + pos;
+    }
+    if (q >= (int64_t)nx * nz) return -1;
+    const int i = (int)(q / nz), k = (int)(q % nz);
+    return ((int64_t)k * ny + pos) * nxp + i;
+}
+
+// Value of a component at field index c in FP64; emag selects |E| computed in
+// FP64 from Ex, Ey, Ez (fdtd.c:3693-3703).
+template <typename real>
+__device__ __forceinline__ double sample_value(const real *F, const real *Ex, const real *Ey, const real *Ez, int emag,
+                                               int64_t c) {
+    if (emag) {
+        const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+        return sqrt(ex * ex + ey * ey + ez * ez);
+    }
+    return (double)F[c];
+}
+
+// One plane as float32 in the reference's slice layout.
 template <typename real>
 __global__ __launch_bounds__(256) void slice_k(const real *F, const real *Ex, const real *Ey, const real *Ez,
                                                int emag, char axis, int pos, int nx, int ny, int nz, int nxp,
                                                float *out) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t c;
-    if (axis == 'z') {
-        if (q >= (int64_t)nx * ny) return;
-        const int i = (int)(q / ny), j = (int)(q % ny);
-        c = ((int64_t)pos * ny + j) * nxp + i;
-    } else if (axis == 'x') {
-        if (q >= (int64_t)ny * nz) return;
-        const int j = (int)(q / nz), k = (int)(q % nz);
-        c = ((int64_t)k * ny + j) * nxp + pos;
-    } else {
-        if (q >= (int64_t)nx * nz) return;
-        const int i = (int)(q / nz), k = (int)(q % nz);
-        c = ((int64_t)k * ny + pos) * nxp + i;
-    }
-    double v;
-    if (emag) {
-        const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
-        v = sqrt(ex * ex + ey * ey + ez * ez);
-    } else {
-        v = (double)F[c];
-    }
-    out[q] = (float)v;
+    const int64_t c = plane_cell(q, axis, pos, nx, ny, nz, nxp);
+    if (c < 0) return;
+    out[q] = (float)sample_value(F, Ex, Ey, Ez, emag, c);
+}
+
+// Streamed single-bin DFT of one plane (frequency_analysis.c:548-569 for one
+// bin): at steps t0 + n*interval, n < nsamp, acc[q] += x_n(q) * w[n] with
+// w[n] = exp(-2 pi i k n / N) / N precomputed in FP64.  ref_float rounds the
+// sample to the float32 a slice file holds.  Runs inside the step graph
+// before the step counter advances, so it sees the step just updated.
+template <typename real>
+__global__ __launch_bounds__(256) void dft_accum(DftArgs a) {
+    const int64_t st = *a.step - a.t0;
+    if (st < 0 || st % a.interval) return;
+    const int64_t n = st / a.interval;
+    if (n >= a.nsamp) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = plane_cell(q, a.axis, a.pos, a.nx, a.ny, a.nz, a.nxp);
+    if (c < 0) return;
+    const real *F = static_cast<const real *>(a.F);
+    const real *Ex = static_cast<const real *>(a.Ex), *Ey = static_cast<const real *>(a.Ey),
+               *Ez = static_cast<const real *>(a.Ez);
+    double x = c < a.ncell ? sample_value(F, Ex, Ey, Ez, a.emag, c) : 0.0;
+    if (a.ref_float) x = (double)(float)x;
+    const double2 w = a.w[n];
+    double2 acc = a.acc[q];
+    acc.x = fma(x, w.x, acc.x);
+    acc.y = fma(x, w.y, acc.y);
+    a.acc[q] = acc;
 }
 
 template <typename real>
@@ -167,6 +204,11 @@ void launch_slice(const real *F, const real *Ex, const real *Ey, const real *Ez,
                        axis, pos, nx, ny, nz, nxp, out);
 }
 
+template <typename real>
+void launch_dft(const DftArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(dft_accum<real>, dim3((unsigned)((a.nplane + 255) / 256)), dim3(256), 0, s, a);
+}
+
 template void launch_step_end<double>(const StepEndArgs &, hipStream_t);
 template void launch_step_end<float>(const StepEndArgs &, hipStream_t);
 template void launch_fill<double>(double *, double, int64_t, hipStream_t);
@@ -178,6 +220,8 @@ template void launch_reduce_e<double>(const double *, const double *, const doub
                                       unsigned long long *, hipStream_t);
 template void launch_reduce_e<float>(const float *, const float *, const float *, int, int, int, int,
                                      unsigned long long *, hipStream_t);
+template void launch_dft<double>(const DftArgs &, hipStream_t);
+template void launch_dft<float>(const DftArgs &, hipStream_t);
 template void launch_slice<double>(const double *, const double *, const double *, const double *, int, char, int,
                                    int, int, int, int, float *, hipStream_t);
 template void launch_slice<float>(const float *, const float *, const float *, const float *, int, char, int, int,

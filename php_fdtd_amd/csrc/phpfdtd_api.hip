@@ -28,6 +28,7 @@ template <typename real> void launch_de(const YeeArgs<real> &, int, hipStream_t)
 template <typename real> void launch_ade(const SlotArgs<real> &, hipStream_t);
 template <typename real> void launch_ion(const SlotArgs<real> &, hipStream_t);
 template <typename real> void launch_step_end(const StepEndArgs &, hipStream_t);
+template <typename real> void launch_dft(const DftArgs &, hipStream_t);
 template <typename T> void launch_fill(T *, T, int64_t, hipStream_t);
 template <typename real>
 void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
@@ -129,6 +130,10 @@ struct phpfdtd_ctx {
     const double *mon_r = nullptr, *mon_v = nullptr;
     unsigned long long *d_diag = nullptr;
     float *d_slice = nullptr;
+    // near-field DFT of one plane
+    bool dft_on = false;
+    DftArgs dft{};
+    double2 *d_dft_w = nullptr, *d_dft_acc = nullptr;
     // plan and graphs
     std::vector<Kernel> plan;
     hipGraphExec_t g1 = nullptr, g16 = nullptr;
@@ -177,6 +182,15 @@ struct phpfdtd_ctx {
     }
 
     size_t fidx(int i, int j, int k) const { return ((size_t)k * ny + j) * nxp + i; }
+
+    // free one allocation made by alloc (after the stream is idle)
+    void release(void *p) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it == allocs.end()) return;
+        allocs.erase(it);
+        hipFree(p);
+    }
 };
 
 static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nslot,
@@ -369,6 +383,15 @@ static void build_plan_t(phpfdtd_ctx *x) {
         }
         k.name = nm;
         x->plan.push_back(k);
+    }
+    if (x->dft_on) {
+        DftArgs da = x->dft;
+        Kernel kf;
+        kf.name = "dft_accum";
+        kf.bytes = (double)da.nplane * (w + 32);
+        kf.flops = (double)da.nplane * 4;
+        kf.fn = [da](hipStream_t s) { launch_dft<real>(da, s); };
+        x->plan.push_back(kf);
     }
     StepEndArgs se{};
     se.step = x->d_step;
@@ -674,6 +697,113 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
     return 0;
 }
 
+// Plane of phpfdtd_read_slice / phpfdtd_set_dft: element count and the plane
+// index actually read (ref_compat z reads pos+1, which may be the zero halo nz).
+static int plane_of(const phpfdtd_ctx *x, char axis, int pos, int ref_compat, size_t *n, int *p) {
+    *p = pos;
+    if (axis == 'z') {
+        if (pos < 0 || pos >= x->nz) return fail(PHPFDTD_EINVAL, "Error: No process contains slice z=%d", pos);
+        *n = (size_t)x->nx * x->ny;
+        if (ref_compat) *p = pos + 1;
+    } else if (axis == 'x') {
+        if (pos < 0 || pos >= x->nx)
+            return fail(PHPFDTD_EINVAL, "Error: Invalid x slice position %d (range: 0-%d)", pos, x->nx - 1);
+        *n = (size_t)x->ny * x->nz;
+    } else if (axis == 'y') {
+        if (pos < 0 || pos >= x->ny)
+            return fail(PHPFDTD_EINVAL, "Error: Invalid y slice position %d (range: 0-%d)", pos, x->ny - 1);
+        *n = (size_t)x->nx * x->nz;
+    } else {
+        return fail(PHPFDTD_EINVAL, "Error: Invalid slice axis '%c'. Must be 'x', 'y', or 'z'.", axis);
+    }
+    return 0;
+}
+
+int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
+                    double f_hz, int *bin) {
+    if (!x || comp < 0 || comp > 9 || interval < 1 || nsamples < 2)
+        return fail(PHPFDTD_EINVAL, "bad arguments (comp %d, interval %d, nsamples %d)", comp, interval, nsamples);
+    size_t n;
+    int p;
+    int rc = plane_of(x, axis, pos, ref_compat, &n, &p);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(x->device));
+    // bin nearest f_hz with the arithmetic of frequency_analysis.c:362-390:
+    // times t_n = (t0 + n*interval)*dt, T = t_{N-1} - t_0, fs = (N-1)/T
+    const int N = nsamples;
+    const double dt = x->g.dt;
+    const double T = (double)(x->host_step + (int64_t)(N - 1) * interval) * dt - (double)x->host_step * dt;
+    const double fs = 1.0 / (T / (N - 1));
+    int k = -1;
+    double best = 1e100;
+    for (int i = 0; i < N / 2; ++i) {
+        const double d = fabs(i * fs / N - f_hz);
+        if (d < best) {
+            best = d;
+            k = i;
+        }
+    }
+    if (k < 0) return fail(PHPFDTD_EINVAL, "Error finding target frequency index");
+    // weights exp(-2 pi i k n / N) / N, angle reduced mod N exactly in integers
+    std::vector<double2> w(N);
+    for (int q = 0; q < N; ++q) {
+        const double th = 2.0 * M_PI * (double)(((int64_t)k * q) % N) / N;
+        w[q].x = cos(th) / N;
+        w[q].y = -sin(th) / N;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->release(x->d_dft_w);
+    x->release(x->d_dft_acc);
+    x->d_dft_w = x->d_dft_acc = nullptr;
+    x->dft_on = false;
+    if ((rc = x->upload((void **)&x->d_dft_w, w))) return rc;
+    if ((rc = x->alloc((void **)&x->d_dft_acc, n * sizeof(double2)))) return rc;
+    DftArgs &a = x->dft;
+    a = DftArgs{};
+    const int emag = comp == PHPFDTD_EMAG;
+    a.F = x->F[emag ? 0 : comp];
+    a.Ex = x->F[0]; a.Ey = x->F[1]; a.Ez = x->F[2];
+    a.emag = emag;
+    a.ref_float = ref_compat ? 1 : 0;
+    a.axis = axis;
+    a.pos = p;
+    a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp;
+    a.ncell = (int64_t)x->ncellp;
+    a.nplane = (int64_t)n;
+    a.t0 = x->host_step;
+    a.interval = interval;
+    a.nsamp = N;
+    a.w = x->d_dft_w;
+    a.acc = x->d_dft_acc;
+    a.step = x->d_step;
+    x->dft_on = true;
+    x->dirty = true;
+    if (bin) *bin = k;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int phpfdtd_read_dft(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
+    if (!x) return fail(PHPFDTD_EINVAL, "null ctx");
+    if (!x->dft_on) return fail(PHPFDTD_ESTATE, "no DFT set (phpfdtd_set_dft)");
+    HIPCHK(hipSetDevice(x->device));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    const DftArgs &a = x->dft;
+    if (n_samples) {
+        const int64_t done = x->host_step - a.t0;  // steps t0 .. host_step-1 have run
+        const int64_t ns = done <= 0 ? 0 : (done - 1) / a.interval + 1;
+        *n_samples = (int)std::min<int64_t>(ns, a.nsamp);
+    }
+    if (!re && !im) return 0;
+    std::vector<double2> h(a.nplane);
+    HIPCHK(hipMemcpy(h.data(), a.acc, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < a.nplane; ++q) {
+        if (re) re[q] = h[q].x;
+        if (im) im[q] = h[q].y;
+    }
+    return 0;
+}
+
 int phpfdtd_run(phpfdtd_ctx *x, int nsteps) {
     if (!x || nsteps < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
     if (x->host_step + nsteps > x->g.tsteps)
@@ -746,27 +876,15 @@ int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
 
 int phpfdtd_read_slice(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, float *out) {
     if (!x || !out || comp < 0 || comp > 9) return fail(PHPFDTD_EINVAL, "bad arguments");
-    HIPCHK(hipSetDevice(x->device));
     size_t n;
-    int p = pos;
-    if (axis == 'z') {
-        if (pos < 0 || pos >= x->nz) return fail(PHPFDTD_EINVAL, "Error: No process contains slice z=%d", pos);
-        n = (size_t)x->nx * x->ny;
-        if (ref_compat) p = pos + 1;
-        if (p >= x->nz) {  // the reference reads its zero upper halo plane
-            memset(out, 0, n * sizeof(float));
-            return 0;
-        }
-    } else if (axis == 'x') {
-        if (pos < 0 || pos >= x->nx)
-            return fail(PHPFDTD_EINVAL, "Error: Invalid x slice position %d (range: 0-%d)", pos, x->nx - 1);
-        n = (size_t)x->ny * x->nz;
-    } else if (axis == 'y') {
-        if (pos < 0 || pos >= x->ny)
-            return fail(PHPFDTD_EINVAL, "Error: Invalid y slice position %d (range: 0-%d)", pos, x->ny - 1);
-        n = (size_t)x->nx * x->nz;
-    } else {
-        return fail(PHPFDTD_EINVAL, "Error: Invalid slice axis '%c'. Must be 'x', 'y', or 'z'.", axis);
+    int p;
+    int rc = plane_of(x, axis, pos, ref_compat, &n, &p);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(x->device));
+    if (axis == 'z' && p >= x->nz) {  // the reference reads its zero upper halo plane
+        HIPCHK(hipStreamSynchronize(x->stream));
+        memset(out, 0, n * sizeof(float));
+        return 0;
     }
     float *buf = x->d_slice;
     const int emag = comp == PHPFDTD_EMAG;

@@ -174,4 +174,19 @@ struct StepEndArgs {
     double dt;
 };
 
+// Streamed single-bin DFT of one plane (phpfdtd_set_dft).
+struct DftArgs {
+    const void *F, *Ex, *Ey, *Ez;  // component sampled (real), and E for |E|
+    int emag, ref_float;
+    char axis;
+    int pos;                       // plane index actually read (z+1 applied)
+    int nx, ny, nz, nxp;
+    int64_t ncell;                 // cells of a field array (plane past the top reads 0)
+    int64_t nplane;                // elements of the plane
+    int64_t t0, interval, nsamp;
+    const double2 *w;              // [nsamp] exp(-2 pi i k n / N) / N
+    double2 *acc;                  // [nplane]
+    const int64_t *step;
+};
+
 }  // namespace phpfdtd

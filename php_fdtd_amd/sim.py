@@ -94,6 +94,27 @@ class Simulation:
                                                          _abi.f32p(out)))
         return out
 
+    def _plane_shape(self, axis):
+        nx, ny, nz = self.shape
+        return {"z": (nx, ny), "x": (ny, nz), "y": (nx, nz)}.get(axis, (1,))
+
+    def set_dft(self, comp, axis, pos, interval, nsamples, f_hz, ref_compat=True):
+        """Streamed single-bin DFT of one plane (phpfdtd_set_dft); returns the bin index."""
+        c = _abi.COMPONENTS[comp] if isinstance(comp, str) else comp
+        k = ctypes.c_int()
+        _abi.check(self.lib, self.lib.phpfdtd_set_dft(self.h, c, axis.encode(), pos, 1 if ref_compat else 0,
+                                                      int(interval), int(nsamples), float(f_hz), ctypes.byref(k)))
+        self._dft_shape = self._plane_shape(axis)
+        return k.value
+
+    def dft(self):
+        """(X, samples accumulated): X complex in the slice layout."""
+        re = np.empty(self._dft_shape, np.float64)
+        im = np.empty(self._dft_shape, np.float64)
+        n = ctypes.c_int()
+        _abi.check(self.lib, self.lib.phpfdtd_read_dft(self.h, _abi.f64p(re), _abi.f64p(im), ctypes.byref(n)))
+        return re + 1j * im, n.value
+
     def monitor(self, max_rows=1 << 20):
         out = np.empty((max_rows, 12), np.float64)
         n = ctypes.c_int()
