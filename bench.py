@@ -122,6 +122,60 @@ def accuracy(name, interval=10):
     }
 
 
+def serving(n, step_scale, seed=4, load=0.8):
+    """Serving mix (grids 64^3..256x256x160, Lorentz / 4-atom / 16-atom
+    materials, source U(600, 1600) cm^-1, steps U(5000, 20000) x step_scale,
+    seeded): the FP64 plain form one request at a time (end to end: build,
+    run, read, destroy), then the FP32 form through the continuous-batching
+    server with every request queued at t=0 (capacity), then with Poisson
+    arrivals at `load` x that capacity (time-to-result)."""
+    from php_fdtd_amd import _abi, serve, workloads
+
+    descs = workloads.serving_stream(n, seed=seed, step_scale=step_scale)
+    cases = [workloads.serving_case(d) for d in descs]
+
+    def requests(arrivals):
+        return [serve.Request(d["id"], sp, med, arrival=a) for d, (sp, med), a in zip(descs, cases, arrivals)]
+
+    work = float(sum(med.size * sp["grid"]["tsteps"] for sp, med in cases))
+    seq_rows, seq_t = [], 0.0
+    for r in requests([0.0] * n):
+        rows, t = serve.run_alone(r, _abi.FP64)
+        seq_rows.append(rows)
+        seq_t += t
+    log("  serving: FP64 one at a time %.2f s" % seq_t)
+    srv = serve.Server(_abi.FP32)
+    sat = srv.serve(requests([0.0] * n))
+    st_sat = serve.summarize(sat)
+    members = [m for _, m, _ in srv.trace]
+    log("  serving: FP32 batched, all queued: %s" % st_sat)
+    rate = load * st_sat["solves_per_s"]
+    arr = np.cumsum([d["gap"] for d in descs]) / rate
+    arr -= arr[0]
+    poi = serve.Server(_abi.FP32).serve(requests(list(arr)))
+    st_poi = serve.summarize(poi)
+    log("  serving: FP32 batched, Poisson %.2f/s: %s" % (rate, st_poi))
+    errs = []
+    by_id = {d["id"]: rows for d, rows in zip(descs, seq_rows)}
+    for r in sat:
+        ref = by_id[r.id][:, 1:10]
+        den = np.linalg.norm(ref)
+        errs.append(float(np.linalg.norm(r.rows[:, 1:10] - ref) / den) if den else 0.0)
+    seq_sps = n / seq_t
+    return {
+        "workload": "serving mix, %d requests (seed %d), steps U(5000, 20000) x %g, FP32 form batched vs FP64 plain "
+                    "one at a time" % (n, seed, step_scale),
+        "cell_updates": work,
+        "fp64_one_at_a_time": {"solves_per_s": round(seq_sps, 4), "gcell_updates_per_s": round(work / seq_t / 1e9, 3),
+                               "total_s": round(seq_t, 3)},
+        "fp32_batched_saturated": dict(st_sat, mean_members=round(float(np.mean(members)), 2),
+                                       max_members=int(max(members))),
+        "fp32_batched_poisson": dict(st_poi, offered_per_s=round(rate, 4), load=load),
+        "solves_ratio_vs_fp64_one_at_a_time": round(st_sat["solves_per_s"] / seq_sps, 3),
+        "per_request_monitor_rel_l2_vs_fp64": {"median": float(np.median(errs)), "max": float(np.max(errs))},
+    }
+
+
 def cpu_baseline(budget_s=25.0):
     """Reference fdtd.c (oracle/_ref/fdtd.x, -Ofast -flto -funroll-loops) under
     mpirun on the host cores, on the same workload scaled down 4x per axis
@@ -186,6 +240,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64")
     ap.add_argument("--accuracy", default="cfg1", help="config for the FP32-vs-FP64 accuracy run ('' = skip)")
+    ap.add_argument("--serving", type=int, default=64, help="requests of the serving mix (0 = skip)")
+    ap.add_argument("--serving-step-scale", type=float, default=0.05)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the domain is resident on a single MI355X and no "
@@ -254,6 +310,8 @@ def main():
             out["fp32_monitor_rel_l2_vs_fp64"] = float(num / den) if den else None
     if args.accuracy:
         out["accuracy"] = accuracy(args.accuracy)
+    if args.serving:
+        out["serving"] = serving(args.serving, args.serving_step_scale)
     if not args.no_cpu:
         try:
             t0 = time.time()

@@ -208,6 +208,34 @@ int phpfdtd_set_dft(phpfdtd_ctx *ctx, int comp, char axis, int pos, int ref_comp
  * accumulated so far. */
 int phpfdtd_read_dft(phpfdtd_ctx *ctx, double *re, double *im, int *n_samples);
 
+/* Batched stepping of several contexts (continuous batching for serving).
+ * No counterpart in the reference, which runs one simulation per process.
+ * A batch advances its member contexts together on the batch's own stream:
+ * each kernel of a step (H, D/E, ADE, lattice per atom count, step epilogue)
+ * is ONE launch over the tiles of all members, each tile reading its member's
+ * argument block, so many small domains fill the GPU like one large one.
+ * Every member computes exactly what it computes alone (same kernels and
+ * arithmetic).  Members keep their own state, sources, monitor and DFT, and
+ * stay readable with the single-context calls (which wait for the batch's
+ * work); phpfdtd_run / phpfdtd_profile / phpfdtd_time_steps on a member
+ * return PHPFDTD_ESTATE.  Membership may change between phpfdtd_batch_run
+ * calls; destroying a member removes it.  Members must share the batch's
+ * precision.  Not thread-safe together with calls on its members. */
+typedef struct phpfdtd_batch phpfdtd_batch;
+int  phpfdtd_batch_create(int precision, phpfdtd_batch **out);
+int  phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *ctx);
+int  phpfdtd_batch_remove(phpfdtd_batch *b, phpfdtd_ctx *ctx);
+/* Enqueue nsteps steps of every member (each member must have nsteps left
+ * before tsteps); returns after enqueue. */
+int  phpfdtd_batch_run(phpfdtd_batch *b, int nsteps);
+/* Wait for the batch's queued work. */
+int  phpfdtd_batch_sync(phpfdtd_batch *b);
+/* Wait until at most `keep` phpfdtd_batch_run calls are still queued (keep=1:
+ * the latest runs on while the host prepares the next). */
+int  phpfdtd_batch_wait(phpfdtd_batch *b, int keep);
+int  phpfdtd_batch_size(const phpfdtd_batch *b);
+void phpfdtd_batch_destroy(phpfdtd_batch *b);
+
 /* Whole component in file order, FP64 (tests, checkpoints). */
 int phpfdtd_read_field(phpfdtd_ctx *ctx, int comp, double *out);
 int phpfdtd_write_field(phpfdtd_ctx *ctx, int comp, const double *in);

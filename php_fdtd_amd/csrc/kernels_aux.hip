@@ -26,6 +26,23 @@ __global__ void step_end(StepEndArgs a) {
     *a.step = st + 1;
 }
 
+// step_end of every domain of a batch: work-group b handles domain b.
+template <typename real>
+__global__ void step_end_batch(const StepEndArgs *args, int n) {
+    if (threadIdx.x != 0 || (int)blockIdx.x >= n) return;
+    const StepEndArgs a = args[blockIdx.x];
+    const int64_t st = *a.step;
+    if (a.mon_on && st < a.ring_rows) {
+        double *row = a.ring + st * 12;
+        row[0] = (double)st * a.dt;
+        for (int q = 0; q < 9; ++q)
+            row[1 + q] = a.mon_c >= 0 ? (double)static_cast<const real *>(a.F[q])[a.mon_c] : 0.0;
+        row[10] = a.mon_ridx >= 0 ? a.mon_r[a.mon_ridx] : 0.0;
+        row[11] = a.mon_ridx >= 0 ? a.mon_v[a.mon_vidx] : 0.0;
+    }
+    *a.step = st + 1;
+}
+
 template <typename T>
 __global__ void fill(T *p, T v, int64_t n) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
@@ -174,6 +191,11 @@ void launch_step_end(const StepEndArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(step_end<real>, dim3(1), dim3(64), 0, s, a);
 }
 
+template <typename real>
+void launch_step_end_batch(const StepEndArgs *args, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(step_end_batch<real>, dim3(n), dim3(64), 0, s, args, n);
+}
+
 template <typename T>
 void launch_fill(T *p, T v, int64_t n, hipStream_t s) {
     if (n <= 0) return;
@@ -211,6 +233,8 @@ void launch_dft(const DftArgs &a, hipStream_t s) {
 
 template void launch_step_end<double>(const StepEndArgs &, hipStream_t);
 template void launch_step_end<float>(const StepEndArgs &, hipStream_t);
+template void launch_step_end_batch<double>(const StepEndArgs *, int, hipStream_t);
+template void launch_step_end_batch<float>(const StepEndArgs *, int, hipStream_t);
 template void launch_fill<double>(double *, double, int64_t, hipStream_t);
 template void launch_fill<float>(float *, float, int64_t, hipStream_t);
 template void launch_fill<unsigned long long>(unsigned long long *, unsigned long long, int64_t, hipStream_t);

@@ -69,10 +69,9 @@ __device__ __forceinline__ void ade_step(const SlotArgs<real> &a, int64_t s, con
     }
 }
 
-// ADE-only materials (natom == 0).
+// ADE-only materials (natom == 0): slot s.
 template <typename real>
-__global__ __launch_bounds__(256) void ade_update(SlotArgs<real> a) {
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void ade_slot(const SlotArgs<real> &a, const int64_t s) {
     if (s >= a.nslot) return;
     const size_t c = a.cells[s];
     const real Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
@@ -91,12 +90,10 @@ __global__ __launch_bounds__(256) void ade_update(SlotArgs<real> a) {
 
 // Plain form: one thread per slot, everything FP64.  NA > 0 fixes natom.
 template <int NA>
-__global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const double *__restrict__ dmm,
-                                                      const double *__restrict__ qezv,
-                                                      const double *__restrict__ qezm, double *__restrict__ rr,
-                                                      double *__restrict__ vv) {
+__device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const double *__restrict__ dmm,
+                                             const double *__restrict__ qezv, const double *__restrict__ qezm,
+                                             double *__restrict__ rr, double *__restrict__ vv, const int64_t s) {
     constexpr int NMAX = NA > 0 ? 3 * NA : 3 * kMaxAtom;
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s >= a.nslot) return;
     const int N3 = NA > 0 ? 3 * NA : 3 * a.natom;
     const size_t c = a.cells[s];
@@ -113,9 +110,9 @@ __global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const 
     for (int q = 0; q < NMAX; ++q) {
         if (q < N3) {
             const int at = q / 3, d = q % 3;
-            P[0] += qezv[at * 9 + 0 + d] * r[q];
-            P[1] += qezv[at * 9 + 3 + d] * r[q];
-            P[2] += qezv[at * 9 + 6 + d] * r[q];
+            P[0] += sld(qezv, at * 9 + 0 + d) * r[q];
+            P[1] += sld(qezv, at * 9 + 3 + d) * r[q];
+            P[2] += sld(qezv, at * 9 + 6 + d) * r[q];
         }
     }
     double Dp[3] = {Dc[0] - P[0], Dc[1] - P[1], Dc[2] - P[2]};
@@ -137,10 +134,11 @@ __global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const 
     for (int q = 0; q < NMAX; ++q) {
         if (q < N3) {
             const int at = q / 3, d = q % 3;
-            double acc = qezm[at * 9 + 0 + d] * Ec[0] + qezm[at * 9 + 3 + d] * Ec[1] + qezm[at * 9 + 6 + d] * Ec[2];
+            double acc = sld(qezm, at * 9 + 0 + d) * Ec[0] + sld(qezm, at * 9 + 3 + d) * Ec[1] +
+                         sld(qezm, at * 9 + 6 + d) * Ec[2];
 #pragma unroll
             for (int jq = 0; jq < NMAX; ++jq)
-                if (jq < N3) acc -= dmm[q * N3 + jq] * r[jq];
+                if (jq < N3) acc -= sld(dmm, q * N3 + jq) * r[jq];
             const double vq = ldnt(vv + tb + q * 64);
             stnt(r[q] + (vq * dt + acc * hdt2), rr + tb + q * 64);
             stnt(vq + acc * dt, vv + tb + q * 64);
@@ -171,14 +169,14 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // requested before group g's MFMA chain.
 // aop[(mt*nk + kt)*64 + l] and qop[mt*64 + l] hold the A operands in lane order.
 template <int NA>
-__global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
-                                                       const float *__restrict__ qop, double *__restrict__ rr,
-                                                       double *__restrict__ vv) {
+__device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const float *__restrict__ aop,
+                                               const float *__restrict__ qop, double *__restrict__ rr,
+                                               double *__restrict__ vv, const int64_t blk) {
     constexpr int NMX = ((3 * NA + 3) / 4 * 4 + 3 + 15) / 16;
     constexpr int NKX = (3 * NA + 3) / 4;
     constexpr int NAX = (3 * NA + 15) / 16;
     const int lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t tile = blk * 4 + (threadIdx.x >> 6);
     const int64_t ntile = (a.nslot + 63) >> 6;
     if (tile >= ntile) return;  // wave-uniform
     const IonLayout L = a.lay;
@@ -309,6 +307,49 @@ __global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const 
 }
 
 template <typename real>
+__global__ __launch_bounds__(256) void ade_update(SlotArgs<real> a) {
+    ade_slot(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <int NA>
+__global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const double *__restrict__ dmm,
+                                                      const double *__restrict__ qezv,
+                                                      const double *__restrict__ qezm, double *__restrict__ rr,
+                                                      double *__restrict__ vv) {
+    ion_f64_slot<NA>(a, dmm, qezv, qezm, rr, vv, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <int NA>
+__global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
+                                                       const float *__restrict__ qop, double *__restrict__ rr,
+                                                       double *__restrict__ vv) {
+    ion_mfma_block<NA>(a, aop, qop, rr, vv, blockIdx.x);
+}
+
+// Batched launches over the slot work of several domains: work-group g is
+// block g - start[map[g]] of domain map[g].
+template <typename real>
+__global__ __launch_bounds__(256) void ade_update_batch(BatchLaunch<SlotArgs<real>> b) {
+    const int rid = (int)sld(b.map, blockIdx.x);
+    const SlotArgs<real> a = batch_args(b, rid);
+    ade_slot(a, (int64_t)(blockIdx.x - sld(b.start, rid)) * 256 + threadIdx.x);
+}
+
+template <int NA>
+__global__ __launch_bounds__(256) void ion_update_f64_batch(BatchLaunch<SlotArgs<double>> b) {
+    const int rid = (int)sld(b.map, blockIdx.x);
+    const SlotArgs<double> a = batch_args(b, rid);
+    ion_f64_slot<NA>(a, a.dmm, a.qezv, a.qezm, a.r, a.v, (int64_t)(blockIdx.x - sld(b.start, rid)) * 256 + threadIdx.x);
+}
+
+template <int NA>
+__global__ __launch_bounds__(256) void ion_update_mfma_batch(BatchLaunch<SlotArgs<float>> b) {
+    const int rid = (int)sld(b.map, blockIdx.x);
+    const SlotArgs<float> a = batch_args(b, rid);
+    ion_mfma_block<NA>(a, a.aop, a.qop, a.r, a.v, blockIdx.x - sld(b.start, rid));
+}
+
+template <typename real>
 void launch_ade(const SlotArgs<real> &a, hipStream_t st) {
     const unsigned nb = (unsigned)((a.nslot + 255) / 256);
     if (nb) hipLaunchKernelGGL(ade_update<real>, dim3(nb), dim3(256), 0, st, a);
@@ -366,7 +407,75 @@ void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
     }
 }
 
+// Work-groups of one domain's slot kernel (the unit of a batched launch).
+template <typename real>
+int64_t slot_blocks(const SlotArgs<real> &a) {
+    if (a.natom > 0 && sizeof(real) == 4) return ((a.nslot + 63) / 64 + 3) / 4;
+    return (a.nslot + 255) / 256;
+}
+
+// Instantiation key of a domain's slot kernel: 0 for ADE-only materials,
+// else the natom template the kernel is compiled for.
+template <typename real>
+int slot_kind(const SlotArgs<real> &a) {
+    if (a.natom == 0) return 0;
+    if (sizeof(real) == 4) return mfma_inst(a.natom);
+    switch (a.natom) {
+        case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return a.natom;
+        default: return kMaxAtom;
+    }
+}
+
+template <typename real>
+void launch_slot_batch(const BatchLaunch<SlotArgs<real>> &b, int kind, hipStream_t st);
+
+template <>
+void launch_slot_batch<double>(const BatchLaunch<SlotArgs<double>> &b, int kind, hipStream_t st) {
+    if (b.total <= 0) return;
+    const dim3 g(b.total), blk(256);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(ade_update_batch<double>, g, blk, 0, st, b); return;
+#define PHPFDTD_ION_CASE(NN) \
+    case NN: hipLaunchKernelGGL(ion_update_f64_batch<NN>, g, blk, 0, st, b); return;
+        PHPFDTD_ION_CASE(1)
+        PHPFDTD_ION_CASE(2)
+        PHPFDTD_ION_CASE(3)
+        PHPFDTD_ION_CASE(4)
+        PHPFDTD_ION_CASE(6)
+        PHPFDTD_ION_CASE(8)
+        PHPFDTD_ION_CASE(12)
+        PHPFDTD_ION_CASE(16)
+#undef PHPFDTD_ION_CASE
+        default: hipLaunchKernelGGL(ion_update_f64_batch<0>, g, blk, 0, st, b); return;
+    }
+}
+
+template <>
+void launch_slot_batch<float>(const BatchLaunch<SlotArgs<float>> &b, int kind, hipStream_t st) {
+    if (b.total <= 0) return;
+    const dim3 g(b.total), blk(256);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(ade_update_batch<float>, g, blk, 0, st, b); return;
+#define PHPFDTD_MFMA_CASE(NN) \
+    case NN: hipLaunchKernelGGL(ion_update_mfma_batch<NN>, g, blk, (size_t)4 * mfma_nk(NN) * 4 * 17 * 4, st, b); return;
+        PHPFDTD_MFMA_CASE(1)
+        PHPFDTD_MFMA_CASE(2)
+        PHPFDTD_MFMA_CASE(3)
+        PHPFDTD_MFMA_CASE(4)
+        PHPFDTD_MFMA_CASE(6)
+        PHPFDTD_MFMA_CASE(8)
+        PHPFDTD_MFMA_CASE(12)
+        PHPFDTD_MFMA_CASE(16)
+        PHPFDTD_MFMA_CASE(kMaxAtom)
+#undef PHPFDTD_MFMA_CASE
+    }
+}
+
 template void launch_ade<double>(const SlotArgs<double> &, hipStream_t);
 template void launch_ade<float>(const SlotArgs<float> &, hipStream_t);
+template int64_t slot_blocks<double>(const SlotArgs<double> &);
+template int64_t slot_blocks<float>(const SlotArgs<float> &);
+template int slot_kind<double>(const SlotArgs<double> &);
+template int slot_kind<float>(const SlotArgs<float> &);
 
 }  // namespace phpfdtd

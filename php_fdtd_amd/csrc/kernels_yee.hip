@@ -38,16 +38,6 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks, int nblocks_pad) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Wavefront-uniform table reads (per-plane z coefficients, per-row y
-// coefficients, source values) go through the scalar cache: the constant
-// address space is read-only for the kernel's lifetime, so these become
-// s_load and wait on their own counter instead of queueing behind the field
-// streams in the in-order vector-memory counter.
-template <typename T>
-__device__ __forceinline__ T sld(const T *p, int64_t idx) {
-    typedef const __attribute__((address_space(4))) T *cptr;
-    return ((cptr)p)[idx];
-}
 
 template <typename real>
 struct Vec;
@@ -82,12 +72,11 @@ __device__ __forceinline__ real shfl_up1(real v) {
     return __shfl_up(v, 1, 64);
 }
 
+// H update of tile t of one domain.
 template <typename real, int R>
-__global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
+__device__ __forceinline__ void h_tile(const YeeArgs<real> &a, const int t) {
     constexpr int VN = Vec<real>::n;
     typedef typename Vec<real>::type V;
-    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
-    if (t < 0) return;
     const int bx = t % a.nbx;
     const int rest = t / a.nbx;
     const int by = rest % a.nby;
@@ -237,22 +226,25 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
     }
 }
 
+// Material table of a domain into LDS (all threads of the work-group).
+template <typename real>
+__device__ __forceinline__ MatCoef<real> *stage_mats(const YeeArgs<real> &a) {
+    extern __shared__ unsigned char lds_raw[];
+    const int nw = a.nmat * (int)(sizeof(MatCoef<real>) / 4);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.mats);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(lds_raw);
+    for (int w = threadIdx.y * 64 + threadIdx.x; w < nw; w += 256) dst[w] = src[w];
+    __syncthreads();
+    return reinterpret_cast<MatCoef<real> *>(lds_raw);
+}
+
+// D update and E of plain cells for tile t of one domain; mats is the LDS
+// copy of the domain's material table.
 template <typename real, int R>
-__global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
+__device__ __forceinline__ void de_tile(const YeeArgs<real> &a, const int t, const MatCoef<real> *mats) {
     constexpr int VN = Vec<real>::n;
     typedef typename Vec<real>::type V;
     typedef typename Vec<real>::ids IDS;
-    extern __shared__ unsigned char lds_raw[];
-    MatCoef<real> *mats = reinterpret_cast<MatCoef<real> *>(lds_raw);
-    {
-        const int nw = nmat * (int)(sizeof(MatCoef<real>) / 4);
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(a.mats);
-        uint32_t *dst = reinterpret_cast<uint32_t *>(lds_raw);
-        for (int w = threadIdx.y * 64 + threadIdx.x; w < nw; w += 256) dst[w] = src[w];
-        __syncthreads();
-    }
-    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
-    if (t < 0) return;
     const int bx = t % a.nbx;
     const int rest = t / a.nbx;
     const int by = rest % a.nby;
@@ -428,15 +420,17 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
                     }
             }
             for (int s = 0; s < a.nsrc; ++s) {
-                if (a.src_k[s] == k && a.src_j[s] == j) {
-                    const int e = a.src_i[s] - x0;
+                const int si = sld(a.src, 4 * s), sj = sld(a.src, 4 * s + 1), sk = sld(a.src, 4 * s + 2),
+                          sc = sld(a.src, 4 * s + 3);
+                if (sk == k && sj == j) {
+                    const int e = si - x0;
                     if (e >= 0 && e < VN) {
                         const real v = sld(a.src_val, st * a.nsrc + s);
 #pragma unroll
                         for (int q = 0; q < VN; ++q)
                             if (q == e) {
-                                if (a.src_c[s] == 0) dx[q] += v;
-                                else if (a.src_c[s] == 1) dy[q] += v;
+                                if (sc == 0) dx[q] += v;
+                                else if (sc == 1) dy[q] += v;
                                 else dz[q] += v;
                             }
                     }
@@ -479,14 +473,51 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a, int nmat) {
 }
 
 template <typename real, int R>
+__global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
+    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
+    if (t < 0) return;
+    h_tile<real, R>(a, t);
+}
+
+template <typename real, int R>
+__global__ __launch_bounds__(256) void de_update(YeeArgs<real> a) {
+    const MatCoef<real> *mats = stage_mats(a);
+    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
+    if (t < 0) return;
+    de_tile<real, R>(a, t, mats);
+}
+
+// Batched launches: the tiles of every domain of a batch in one grid.  The
+// XCD renumbering runs over the concatenated tiles.
+
+template <typename real, int R>
+__global__ __launch_bounds__(256) void h_update_batch(BatchLaunch<YeeArgs<real>> b) {
+    const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
+    if (g < 0) return;
+    const int rid = (int)sld(b.map, g);
+    const YeeArgs<real> a = batch_args(b, rid);
+    h_tile<real, R>(a, g - sld(b.start, rid));
+}
+
+template <typename real, int R>
+__global__ __launch_bounds__(256) void de_update_batch(BatchLaunch<YeeArgs<real>> b) {
+    const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
+    if (g < 0) return;  // whole work-group
+    const int rid = (int)sld(b.map, g);
+    const YeeArgs<real> a = batch_args(b, rid);
+    const MatCoef<real> *mats = stage_mats(a);
+    de_tile<real, R>(a, g - sld(b.start, rid), mats);
+}
+
+template <typename real, int R>
 static void launch_h_r(const YeeArgs<real> &a, hipStream_t s) {
     hipLaunchKernelGGL((h_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), 0, s, a);
 }
 
 template <typename real, int R>
-static void launch_de_r(const YeeArgs<real> &a, int nmat, hipStream_t s) {
-    const size_t lds = (size_t)nmat * sizeof(MatCoef<real>);
-    hipLaunchKernelGGL((de_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a, nmat);
+static void launch_de_r(const YeeArgs<real> &a, hipStream_t s) {
+    const size_t lds = (size_t)a.nmat * sizeof(MatCoef<real>);
+    hipLaunchKernelGGL((de_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a);
 }
 
 template <typename real>
@@ -499,17 +530,34 @@ void launch_h(const YeeArgs<real> &a, hipStream_t s) {
 }
 
 template <typename real>
-void launch_de(const YeeArgs<real> &a, int nmat, hipStream_t s) {
+void launch_de(const YeeArgs<real> &a, hipStream_t s) {
     switch (a.rows) {
-        case 1: launch_de_r<real, 1>(a, nmat, s); break;
-        case 2: launch_de_r<real, 2>(a, nmat, s); break;
-        default: launch_de_r<real, 4>(a, nmat, s); break;
+        case 1: launch_de_r<real, 1>(a, s); break;
+        case 2: launch_de_r<real, 2>(a, s); break;
+        default: launch_de_r<real, 4>(a, s); break;
     }
+}
+
+// Batched H and D/E over the domains of a batch (one y row per wavefront).
+template <typename real>
+void launch_h_batch(const BatchLaunch<YeeArgs<real>> &b, hipStream_t s) {
+    if (b.total > 0) hipLaunchKernelGGL((h_update_batch<real, 1>), dim3(b.total_pad), dim3(64, kBlockY), 0, s, b);
+}
+
+template <typename real>
+void launch_de_batch(const BatchLaunch<YeeArgs<real>> &b, int max_nmat, hipStream_t s) {
+    const size_t lds = (size_t)max_nmat * sizeof(MatCoef<real>);
+    if (b.total > 0)
+        hipLaunchKernelGGL((de_update_batch<real, 1>), dim3(b.total_pad), dim3(64, kBlockY), lds, s, b);
 }
 
 template void launch_h<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_h<float>(const YeeArgs<float> &, hipStream_t);
-template void launch_de<double>(const YeeArgs<double> &, int, hipStream_t);
-template void launch_de<float>(const YeeArgs<float> &, int, hipStream_t);
+template void launch_de<double>(const YeeArgs<double> &, hipStream_t);
+template void launch_de<float>(const YeeArgs<float> &, hipStream_t);
+template void launch_h_batch<double>(const BatchLaunch<YeeArgs<double>> &, hipStream_t);
+template void launch_h_batch<float>(const BatchLaunch<YeeArgs<float>> &, hipStream_t);
+template void launch_de_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);
+template void launch_de_batch<float>(const BatchLaunch<YeeArgs<float>> &, int, hipStream_t);
 
 }  // namespace phpfdtd

@@ -24,7 +24,13 @@
 
 namespace phpfdtd {
 template <typename real> void launch_h(const YeeArgs<real> &, hipStream_t);
-template <typename real> void launch_de(const YeeArgs<real> &, int, hipStream_t);
+template <typename real> void launch_de(const YeeArgs<real> &, hipStream_t);
+template <typename real> void launch_h_batch(const BatchLaunch<YeeArgs<real>> &, hipStream_t);
+template <typename real> void launch_de_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
+template <typename real> void launch_slot_batch(const BatchLaunch<SlotArgs<real>> &, int, hipStream_t);
+template <typename real> int64_t slot_blocks(const SlotArgs<real> &);
+template <typename real> int slot_kind(const SlotArgs<real> &);
+template <typename real> void launch_step_end_batch(const StepEndArgs *, int, hipStream_t);
 template <typename real> void launch_ade(const SlotArgs<real> &, hipStream_t);
 template <typename real> void launch_ion(const SlotArgs<real> &, hipStream_t);
 template <typename real> void launch_step_end(const StepEndArgs &, hipStream_t);
@@ -74,6 +80,8 @@ struct Special {
     double *d_r = nullptr, *d_v = nullptr;
     double *d_dmm = nullptr, *d_qezv = nullptr, *d_qezm = nullptr;
     float *d_aop = nullptr, *d_qop = nullptr;  // MFMA operand tables (precision form)
+    SlotArgs<double> sa64{};                    // argument block of its slot kernel (current plan)
+    SlotArgs<float> sa32{};
 };
 
 struct Kernel {
@@ -118,7 +126,7 @@ struct phpfdtd_ctx {
     int yee_rows = 1, yee_kchunk = 16;  // Yee tile: y rows per wavefront, z planes per chunk
     // sources
     int nsrc = 0;
-    int src_i[kMaxSrc] = {}, src_j[kMaxSrc] = {}, src_k[kMaxSrc] = {}, src_c[kMaxSrc] = {};
+    int *d_src = nullptr;  // [kMaxSrc][4] (i, j, k, component)
     void *d_src_val = nullptr;
     // step counter and monitor ring
     int64_t *d_step = nullptr;
@@ -134,6 +142,13 @@ struct phpfdtd_ctx {
     bool dft_on = false;
     DftArgs dft{};
     double2 *d_dft_w = nullptr, *d_dft_acc = nullptr;
+    // argument blocks of the current plan (batched launches read these)
+    YeeArgs<double> yee64{};
+    YeeArgs<float> yee32{};
+    StepEndArgs se{};
+    // batch membership: the batch's stream runs this context's steps
+    phpfdtd_batch *batch = nullptr;
+    hipEvent_t ev = nullptr;  // marks this stream's latest work for the batch stream
     // plan and graphs
     std::vector<Kernel> plan;
     hipGraphExec_t g1 = nullptr, g16 = nullptr;
@@ -146,6 +161,7 @@ struct phpfdtd_ctx {
         if (stream) (void)hipStreamSynchronize(stream);
         if (g1) hipGraphExecDestroy(g1);
         if (g16) hipGraphExecDestroy(g16);
+        if (ev) hipEventDestroy(ev);
         for (void *p : allocs) hipFree(p);
         if (stream) hipStreamDestroy(stream);
     }
@@ -272,6 +288,13 @@ static void mfma_tables(const Special &sp, std::vector<float> *aop, std::vector<
         }
 }
 
+template <typename real> static YeeArgs<real> &args_of(phpfdtd_ctx *x);
+template <> YeeArgs<double> &args_of<double>(phpfdtd_ctx *x) { return x->yee64; }
+template <> YeeArgs<float> &args_of<float>(phpfdtd_ctx *x) { return x->yee32; }
+template <typename real> static SlotArgs<real> &slot_args_of(Special &sp);
+template <> SlotArgs<double> &slot_args_of<double>(Special &sp) { return sp.sa64; }
+template <> SlotArgs<float> &slot_args_of<float>(Special &sp) { return sp.sa32; }
+
 template <typename real>
 static void build_plan_t(phpfdtd_ctx *x) {
     x->plan.clear();
@@ -295,6 +318,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         axes[d]->slot = x->d_slot[d];
     }
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp; a.L = x->L;
+    a.nmat = x->nmat;
     // a work-group is 4 wavefronts, each `rows` y rows x 64 lanes x one
     // 16-byte vector of x cells, marching over a short chunk of z planes
     // (streams of many short-lived wavefronts run closest to the HBM peak)
@@ -310,9 +334,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
     a.nblocks = bxy * nbz;
     a.nblocks_pad = (a.nblocks + 7) / 8 * 8;
     a.nsrc = x->nsrc;
-    for (int s = 0; s < x->nsrc; ++s) {
-        a.src_i[s] = x->src_i[s]; a.src_j[s] = x->src_j[s]; a.src_k[s] = x->src_k[s]; a.src_c[s] = x->src_c[s];
-    }
+    a.src = x->d_src;
     a.src_val = (const real *)x->d_src_val;
     a.step = x->d_step;
 
@@ -339,8 +361,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
     kd.bytes = cells * (6 * w + 1) + (cells - special_cells) * 3 * w + icells * 3 * w + lossy_int * 3 * w +
                pml_inc(nx - 2, ny - 2, nz) * 4 * w;
     kd.flops = icells * 18 + cells * 15;
-    const int nmat = x->nmat;
-    kd.fn = [a, nmat](hipStream_t s) { launch_de<real>(a, nmat, s); };
+    kd.fn = [a](hipStream_t s) { launch_de<real>(a, s); };
+    args_of<real>(x) = a;
     x->plan.push_back(kd);
 
     for (auto &sp : x->specials) {
@@ -383,6 +405,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         }
         k.name = nm;
         x->plan.push_back(k);
+        slot_args_of<real>(sp) = sa;
     }
     if (x->dft_on) {
         DftArgs da = x->dft;
@@ -405,17 +428,24 @@ static void build_plan_t(phpfdtd_ctx *x) {
     se.mon_ridx = x->mon_ridx;
     se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
+    x->se = se;
     Kernel ke;
     ke.name = "step_end";
     ke.fn = [se](hipStream_t s) { launch_step_end<real>(se, s); };
     x->plan.push_back(ke);
 }
 
+static void build_plan(phpfdtd_ctx *x) {
+    if (x->prec == PHPFDTD_FP64) build_plan_t<double>(x);
+    else build_plan_t<float>(x);
+}
+
+static void touch(phpfdtd_ctx *x);
+
 static int rebuild(phpfdtd_ctx *x) {
     if (x->g1) { hipGraphExecDestroy(x->g1); x->g1 = nullptr; }
     if (x->g16) { hipGraphExecDestroy(x->g16); x->g16 = nullptr; }
-    if (x->prec == PHPFDTD_FP64) build_plan_t<double>(x);
-    else build_plan_t<float>(x);
+    build_plan(x);
     x->dirty = false;
     if (!x->use_graph) return 0;
     for (int which = 0; which < 2; ++which) {
@@ -616,6 +646,7 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     if ((rc = x->alloc((void **)&x->d_ring, (size_t)std::max(1, g->tsteps) * 12 * 8))) return rc;
     if ((rc = x->alloc((void **)&x->d_diag, 64))) return rc;
     if ((rc = x->alloc(&x->d_src_val, x->rs))) return rc;
+    if ((rc = x->alloc((void **)&x->d_src, 4 * kMaxSrc * sizeof(int)))) return rc;
     {
         const size_t mx = std::max({(size_t)x->nx * x->ny, (size_t)x->ny * x->nz, (size_t)x->nx * x->nz});
         if ((rc = x->alloc((void **)&x->d_slice, mx * sizeof(float)))) return rc;
@@ -625,7 +656,11 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     return 0;
 }
 
-void phpfdtd_destroy(phpfdtd_ctx *x) { delete x; }
+void phpfdtd_destroy(phpfdtd_ctx *x) {
+    if (!x) return;
+    if (x->batch) phpfdtd_batch_remove(x->batch, x);
+    delete x;
+}
 
 int64_t phpfdtd_step(const phpfdtd_ctx *x) { return x ? x->host_step : -1; }
 
@@ -655,14 +690,17 @@ int phpfdtd_set_sources(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
         x->allocs.erase(it);
         hipFree(old);
     }
-    x->nsrc = na;
+    std::vector<int> pos(4 * kMaxSrc, 0);
     for (int s = 0; s < na; ++s) {
-        x->src_i[s] = act[s].x;
-        x->src_j[s] = act[s].y;
-        x->src_k[s] = act[s].z;
-        x->src_c[s] = act[s].component - 'x';
+        pos[4 * s] = act[s].x;
+        pos[4 * s + 1] = act[s].y;
+        pos[4 * s + 2] = act[s].z;
+        pos[4 * s + 3] = act[s].component - 'x';
     }
-    x->dirty = true;
+    HIPCHK(hipMemcpyAsync(x->d_src, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->nsrc = na;
+    touch(x);
     return 0;
 }
 
@@ -693,7 +731,7 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
         x->mon_r = x->specials[sp].d_r;
         x->mon_v = x->specials[sp].d_v;
     }
-    x->dirty = true;
+    touch(x);
     return 0;
 }
 
@@ -777,7 +815,7 @@ int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat
     a.acc = x->d_dft_acc;
     a.step = x->d_step;
     x->dft_on = true;
-    x->dirty = true;
+    touch(x);
     if (bin) *bin = k;
     HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
@@ -806,6 +844,7 @@ int phpfdtd_read_dft(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
 
 int phpfdtd_run(phpfdtd_ctx *x, int nsteps) {
     if (!x || nsteps < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    if (x->batch) return fail(PHPFDTD_ESTATE, "context is a batch member; step it with phpfdtd_batch_run");
     if (x->host_step + nsteps > x->g.tsteps)
         return fail(PHPFDTD_EINVAL, "run past tsteps (%lld + %d > %d)", (long long)x->host_step, nsteps, x->g.tsteps);
     HIPCHK(hipSetDevice(x->device));
@@ -1066,6 +1105,7 @@ int phpfdtd_write_state(phpfdtd_ctx *x, int mat, int kind, const double *in) {
 int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out, int *n_out) {
     if (!x || nsteps <= 0 || !n_out) return fail(PHPFDTD_EINVAL, "bad arguments");
     if (x->host_step + nsteps > x->g.tsteps) return fail(PHPFDTD_EINVAL, "profile past tsteps");
+    if (x->batch) return fail(PHPFDTD_ESTATE, "context is a batch member");
     HIPCHK(hipSetDevice(x->device));
     if (x->dirty) {
         int rc = rebuild(x);
@@ -1105,6 +1145,7 @@ int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out,
 
 int phpfdtd_time_steps(phpfdtd_ctx *x, int nsteps, double *ms) {
     if (!x || !ms || nsteps <= 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    if (x->batch) return fail(PHPFDTD_ESTATE, "context is a batch member");
     HIPCHK(hipSetDevice(x->device));
     if (x->dirty) {
         int rc = rebuild(x);
@@ -1132,6 +1173,313 @@ int phpfdtd_struct_sizes(int *sizes, int n) {
                       (int)sizeof(phpfdtd_diag), (int)sizeof(phpfdtd_ktime)};
     for (int q = 0; q < n && q < 5; ++q) sizes[q] = s[q];
     return 5;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: one launch per kernel kind over the tiles of all member contexts.
+// The per-launch tables (argument blocks, tile -> member map, first tile of
+// each member) live in one device buffer, rebuilt when membership or a
+// member's configuration changes.  The upload is stream-ordered on the batch
+// stream, so it lands after every kernel already queued that reads the old
+// tables; two pinned staging buffers let the host build the next tables while
+// the previous upload is still pending.
+struct phpfdtd_batch {
+    int prec = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<phpfdtd_ctx *> members;
+    bool dirty = true;
+    // device tables
+    unsigned char *d_tab = nullptr;
+    size_t d_cap = 0;
+    unsigned char *h_stage[2] = {nullptr, nullptr};
+    size_t h_cap[2] = {0, 0};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_next = 0;
+    // launches of one step (pointers into d_tab)
+    BatchLaunch<YeeArgs<double>> yee64{};
+    BatchLaunch<YeeArgs<float>> yee32{};
+    struct SlotGroup {
+        int kind = 0;
+        BatchLaunch<SlotArgs<double>> b64{};
+        BatchLaunch<SlotArgs<float>> b32{};
+    };
+    std::vector<SlotGroup> groups;
+    const StepEndArgs *d_se = nullptr;
+    int max_nmat = 1;
+    // completion events of queued phpfdtd_batch_run calls, oldest first
+    std::vector<hipEvent_t> inflight, spare;
+
+    ~phpfdtd_batch() {
+        if (!stream) return;
+        (void)hipSetDevice(device);
+        (void)hipStreamSynchronize(stream);
+        for (int q = 0; q < 2; ++q) {
+            if (h_stage[q]) (void)hipHostFree(h_stage[q]);
+            if (stage_ev[q]) (void)hipEventDestroy(stage_ev[q]);
+        }
+        if (d_tab) (void)hipFree(d_tab);
+        if (ev) (void)hipEventDestroy(ev);
+        for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(stream);
+    }
+};
+
+// Mark a context's plan stale, and its batch's tables with it.
+static void touch(phpfdtd_ctx *x) {
+    x->dirty = true;
+    if (x->batch) x->batch->dirty = true;
+}
+
+namespace {
+// Host image of the batch tables: byte blob with 256-byte aligned sections.
+struct Blob {
+    std::vector<unsigned char> b;
+    size_t put(const void *p, size_t n) {
+        const size_t off = (b.size() + 255) & ~(size_t)255;
+        b.resize(off + n);
+        if (n) memcpy(b.data() + off, p, n);
+        return off;
+    }
+};
+
+template <typename A>
+struct LaunchImage {
+    std::vector<A> args;
+    std::vector<uint32_t> map;
+    std::vector<int> start;
+    size_t o_args = 0, o_map = 0, o_start = 0;
+    int total = 0, total_pad = 0;
+    void add(const A &a, int64_t tiles) {
+        start.push_back((int)map.size());
+        const uint32_t rid = (uint32_t)args.size();
+        args.push_back(a);
+        map.insert(map.end(), (size_t)tiles, rid);
+    }
+    void place(Blob &bl, int pad_to) {
+        total = (int)map.size();
+        total_pad = (total + pad_to - 1) / pad_to * pad_to;
+        o_args = bl.put(args.data(), args.size() * sizeof(A));
+        o_map = bl.put(map.data(), map.size() * sizeof(uint32_t));
+        o_start = bl.put(start.data(), start.size() * sizeof(int));
+    }
+    BatchLaunch<A> bind(unsigned char *base) const {
+        BatchLaunch<A> l;
+        l.args = reinterpret_cast<const A *>(base + o_args);
+        l.map = reinterpret_cast<const uint32_t *>(base + o_map);
+        l.start = reinterpret_cast<const int *>(base + o_start);
+        l.total = total;
+        l.total_pad = total_pad;
+        return l;
+    }
+};
+}  // namespace
+
+template <typename real>
+static int batch_tables_t(phpfdtd_batch *b) {
+    LaunchImage<YeeArgs<real>> yee;
+    std::vector<int> kinds;
+    std::vector<LaunchImage<SlotArgs<real>>> slot;
+    std::vector<StepEndArgs> se;
+    b->max_nmat = 1;
+    for (phpfdtd_ctx *x : b->members) {
+        build_plan(x);
+        const YeeArgs<real> &a = args_of<real>(x);
+        yee.add(a, a.nblocks);
+        b->max_nmat = std::max(b->max_nmat, x->nmat);
+        for (auto &sp : x->specials) {
+            const SlotArgs<real> &sa = slot_args_of<real>(sp);
+            const int kind = slot_kind(sa);
+            size_t g = 0;
+            while (g < kinds.size() && kinds[g] != kind) ++g;
+            if (g == kinds.size()) {
+                kinds.push_back(kind);
+                slot.emplace_back();
+            }
+            slot[g].add(sa, slot_blocks(sa));
+        }
+        se.push_back(x->se);
+    }
+    Blob bl;
+    yee.place(bl, 8);
+    for (auto &l : slot) l.place(bl, 1);
+    const size_t o_se = bl.put(se.data(), se.size() * sizeof(StepEndArgs));
+    const size_t n = bl.b.size();
+    // device buffer (grown only after the stream has drained the old one)
+    if (n > b->d_cap) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (b->d_tab) HIPCHK(hipFree(b->d_tab));
+        b->d_tab = nullptr;
+        const size_t cap = std::max(n * 2, (size_t)1 << 20);
+        hipError_t e = hipMalloc((void **)&b->d_tab, cap);
+        if (e != hipSuccess) {
+            b->d_cap = 0;
+            return fail(PHPFDTD_ENOMEM_HBM, "batch tables: hipMalloc(%zu): %s", cap, hipGetErrorString(e));
+        }
+        b->d_cap = cap;
+    }
+    // pinned staging buffer whose previous upload has completed
+    const int q = b->stage_next;
+    b->stage_next ^= 1;
+    HIPCHK(hipEventSynchronize(b->stage_ev[q]));
+    if (n > b->h_cap[q]) {
+        if (b->h_stage[q]) HIPCHK(hipHostFree(b->h_stage[q]));
+        b->h_stage[q] = nullptr;
+        const size_t cap = std::max(n * 2, (size_t)1 << 20);
+        HIPCHK(hipHostMalloc((void **)&b->h_stage[q], cap, hipHostMallocDefault));
+        b->h_cap[q] = cap;
+    }
+    memcpy(b->h_stage[q], bl.b.data(), n);
+    HIPCHK(hipMemcpyAsync(b->d_tab, b->h_stage[q], n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipEventRecord(b->stage_ev[q], b->stream));
+    if constexpr (sizeof(real) == 8) b->yee64 = yee.bind(b->d_tab);
+    else b->yee32 = yee.bind(b->d_tab);
+    b->groups.clear();
+    for (size_t g = 0; g < kinds.size(); ++g) {
+        phpfdtd_batch::SlotGroup sg;
+        sg.kind = kinds[g];
+        if constexpr (sizeof(real) == 8) sg.b64 = slot[g].bind(b->d_tab);
+        else sg.b32 = slot[g].bind(b->d_tab);
+        b->groups.push_back(sg);
+    }
+    b->d_se = reinterpret_cast<const StepEndArgs *>(b->d_tab + o_se);
+    b->dirty = false;
+    return 0;
+}
+
+template <typename real>
+static void batch_step_t(phpfdtd_batch *b) {
+    const BatchLaunch<YeeArgs<real>> &y = [&]() -> const BatchLaunch<YeeArgs<real>> & {
+        if constexpr (sizeof(real) == 8) return b->yee64;
+        else return b->yee32;
+    }();
+    launch_h_batch<real>(y, b->stream);
+    launch_de_batch<real>(y, b->max_nmat, b->stream);
+    for (auto &g : b->groups) {
+        if constexpr (sizeof(real) == 8) launch_slot_batch<real>(g.b64, g.kind, b->stream);
+        else launch_slot_batch<real>(g.b32, g.kind, b->stream);
+    }
+    for (phpfdtd_ctx *x : b->members)
+        if (x->dft_on) launch_dft<real>(x->dft, b->stream);
+    launch_step_end_batch<real>(b->d_se, (int)b->members.size(), b->stream);
+}
+
+extern "C" {
+
+int phpfdtd_batch_create(int precision, phpfdtd_batch **out) {
+    if (!out || (precision != PHPFDTD_FP64 && precision != PHPFDTD_FP32)) return fail(PHPFDTD_EINVAL, "bad arguments");
+    *out = nullptr;
+    std::unique_ptr<phpfdtd_batch> b(new phpfdtd_batch());
+    b->prec = precision;
+    HIPCHK(hipGetDevice(&b->device));
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
+    for (int q = 0; q < 2; ++q) {
+        HIPCHK(hipEventCreateWithFlags(&b->stage_ev[q], hipEventDisableTiming));
+        HIPCHK(hipEventRecord(b->stage_ev[q], b->stream));
+    }
+    *out = b.release();
+    return 0;
+}
+
+int phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *x) {
+    if (!b || !x) return fail(PHPFDTD_EINVAL, "null argument");
+    if (x->batch) return fail(PHPFDTD_ESTATE, "context is already in a batch");
+    if (x->prec != b->prec) return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->prec, b->prec);
+    if (x->device != b->device) return fail(PHPFDTD_EINVAL, "context is on device %d, batch on %d", x->device, b->device);
+    if (x->yee_rows != 1) return fail(PHPFDTD_EINVAL, "batched launches use one y row per wavefront");
+    HIPCHK(hipSetDevice(x->device));
+    if (!x->ev) HIPCHK(hipEventCreateWithFlags(&x->ev, hipEventDisableTiming));
+    b->members.push_back(x);
+    x->batch = b;
+    b->dirty = true;
+    return 0;
+}
+
+int phpfdtd_batch_remove(phpfdtd_batch *b, phpfdtd_ctx *x) {
+    if (!b || !x) return fail(PHPFDTD_EINVAL, "null argument");
+    auto it = std::find(b->members.begin(), b->members.end(), x);
+    if (it == b->members.end()) return fail(PHPFDTD_EINVAL, "context is not a member of this batch");
+    b->members.erase(it);
+    x->batch = nullptr;
+    x->dirty = true;  // its graphs were captured before joining or are stale
+    b->dirty = true;
+    return 0;
+}
+
+int phpfdtd_batch_run(phpfdtd_batch *b, int nsteps) {
+    if (!b || nsteps < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    for (phpfdtd_ctx *x : b->members)
+        if (x->host_step + nsteps > x->g.tsteps)
+            return fail(PHPFDTD_EINVAL, "member runs past tsteps (%lld + %d > %d)", (long long)x->host_step, nsteps,
+                        x->g.tsteps);
+    if (b->members.empty() || nsteps == 0) return 0;
+    HIPCHK(hipSetDevice(b->device));
+    if (b->dirty) {
+        int rc = b->prec == PHPFDTD_FP64 ? batch_tables_t<double>(b) : batch_tables_t<float>(b);
+        if (rc) return rc;
+    }
+    // members' own queued work (uploads, state writes) comes first
+    for (phpfdtd_ctx *x : b->members) {
+        HIPCHK(hipEventRecord(x->ev, x->stream));
+        HIPCHK(hipStreamWaitEvent(b->stream, x->ev, 0));
+    }
+    for (int t = 0; t < nsteps; ++t) {
+        if (b->prec == PHPFDTD_FP64) batch_step_t<double>(b);
+        else batch_step_t<float>(b);
+    }
+    HIPCHK(hipGetLastError());
+    // later work on a member's stream (reads, syncs) waits for these steps
+    HIPCHK(hipEventRecord(b->ev, b->stream));
+    hipEvent_t done;
+    if (!b->spare.empty()) {
+        done = b->spare.back();
+        b->spare.pop_back();
+    } else {
+        HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(done, b->stream));
+    b->inflight.push_back(done);
+    for (phpfdtd_ctx *x : b->members) {
+        HIPCHK(hipStreamWaitEvent(x->stream, b->ev, 0));
+        x->host_step += nsteps;
+    }
+    return 0;
+}
+
+int phpfdtd_batch_sync(phpfdtd_batch *b) {
+    if (!b) return fail(PHPFDTD_EINVAL, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (hipEvent_t e : b->inflight) b->spare.push_back(e);
+    b->inflight.clear();
+    return 0;
+}
+
+int phpfdtd_batch_wait(phpfdtd_batch *b, int keep) {
+    if (!b || keep < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(b->device));
+    while ((int)b->inflight.size() > keep) {
+        HIPCHK(hipEventSynchronize(b->inflight.front()));
+        b->spare.push_back(b->inflight.front());
+        b->inflight.erase(b->inflight.begin());
+    }
+    return 0;
+}
+
+int phpfdtd_batch_size(const phpfdtd_batch *b) { return b ? (int)b->members.size() : 0; }
+
+void phpfdtd_batch_destroy(phpfdtd_batch *b) {
+    if (!b) return;
+    for (phpfdtd_ctx *x : b->members) {
+        x->batch = nullptr;
+        x->dirty = true;
+    }
+    delete b;
 }
 
 }  // extern "C"

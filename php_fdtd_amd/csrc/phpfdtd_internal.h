@@ -66,11 +66,13 @@ struct YeeArgs {
     const MatCoef<real> *mats;
     Axis<real> ax, ay, az;
     int nx, ny, nz, nxp, L;
+    int nmat;  // entries of mats
     int rows;  // y rows per wavefront (1, 2 or 4)
     int kchunk, nbx, nby, nbz, nblocks, nblocks_pad;
-    // sources: value*dt per step in src_val[step*nsrc + s]
+    // sources: cell and component (i, j, k, 0|1|2) in src[4s..4s+3], value*dt
+    // per step in src_val[step*nsrc + s]
     int nsrc;
-    int src_i[kMaxSrc], src_j[kMaxSrc], src_k[kMaxSrc], src_c[kMaxSrc];
+    const int *src;
     const real *src_val;
     const int64_t *step;
 };
@@ -173,6 +175,40 @@ struct StepEndArgs {
     int64_t mon_ridx, mon_vidx;   // offsets of (slot, atom*3+dir) in r and v, -1 = no R/V
     double dt;
 };
+
+// One launch over the work of several domains (a batch): tile g of the grid
+// belongs to domain map[g] and is tile g - start[map[g]] of that domain.
+template <typename A>
+struct BatchLaunch {
+    const A *args;          // [ndomain] argument blocks
+    const uint32_t *map;    // [total]
+    const int *start;       // [ndomain]
+    int total, total_pad;   // tiles, and grid size (multiple of 8 for the Yee kernels)
+};
+
+// Wavefront-uniform reads (coefficient tables, source values, batch tables)
+// go through the scalar cache: the constant address space is read-only for
+// the kernel's lifetime, so these become s_load and wait on their own counter
+// instead of queueing behind the field streams in the vector-memory counter.
+template <typename T>
+__device__ __forceinline__ T sld(const T *p, int64_t idx) {
+    typedef const __attribute__((address_space(4))) T *cptr;
+    return ((cptr)p)[idx];
+}
+
+// Argument block of domain rid of a batched launch (wavefront-uniform, so it
+// is loaded through the scalar cache into scalar registers).
+template <typename A>
+__device__ __forceinline__ A batch_args(const BatchLaunch<A> &b, int rid) {
+    static_assert(sizeof(A) % 4 == 0, "argument blocks are copied as dwords");
+    typedef const __attribute__((address_space(4))) uint32_t *cptr;
+    const cptr src = (cptr)(b.args + rid);
+    A out;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(A) / 4); ++i) dst[i] = src[i];
+    return out;
+}
 
 // Streamed single-bin DFT of one plane (phpfdtd_set_dft).
 struct DftArgs {

@@ -185,3 +185,47 @@ def footprint_bytes(spec, medium, precision=_abi.FP64, lib=None):
     _abi.check(lib, lib.phpfdtd_footprint_bytes(ctypes.byref(grid), mats.arr, mats.n, _abi.i32p(med),
                                                 ctypes.byref(out)))
     return out.value
+
+
+class Batch:
+    """A phpfdtd_batch: member Simulations stepped together, one launch per
+    kernel kind over all members' tiles (see include/phpfdtd.h)."""
+
+    def __init__(self, precision=_abi.FP64, lib=None):
+        self.lib = lib or _abi.load_library()
+        h = ctypes.c_void_p()
+        _abi.check(self.lib, self.lib.phpfdtd_batch_create(int(precision), ctypes.byref(h)))
+        self.h = h
+        self.members = []
+
+    def add(self, sim):
+        _abi.check(self.lib, self.lib.phpfdtd_batch_add(self.h, sim.h))
+        self.members.append(sim)
+
+    def remove(self, sim):
+        _abi.check(self.lib, self.lib.phpfdtd_batch_remove(self.h, sim.h))
+        self.members.remove(sim)
+
+    def run(self, n):
+        _abi.check(self.lib, self.lib.phpfdtd_batch_run(self.h, int(n)))
+
+    def wait(self, keep=0):
+        _abi.check(self.lib, self.lib.phpfdtd_batch_wait(self.h, int(keep)))
+
+    def sync(self):
+        _abi.check(self.lib, self.lib.phpfdtd_batch_sync(self.h))
+
+    def __len__(self):
+        return self.lib.phpfdtd_batch_size(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.phpfdtd_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -244,6 +244,59 @@ def config3(scale=1):
 
 CONFIGS = {"cfg0": config0, "cfg1": config1, "cfg2": config2, "cfg3": config3}
 
+# serving mix (cfg4)
+SERVE_GRIDS = [(64, 64, 64), (96, 96, 96), (128, 128, 128), (192, 192, 128), (256, 256, 160)]
+SERVE_KINDS = ("lorentz", "lattice4", "lattice16")
+
+
+def serving_stream(n, seed=4, step_scale=1.0):
+    """n seeded request descriptors of the serving mix: grid, material kind,
+    source wavenumber U(600, 1600) cm^-1, steps U(5000, 20000) x step_scale,
+    and a unit-rate exponential inter-arrival gap (a Poisson stream once
+    divided by the offered rate)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(dict(id=i, shape=SERVE_GRIDS[int(rng.integers(len(SERVE_GRIDS)))],
+                        kind=SERVE_KINDS[int(rng.integers(len(SERVE_KINDS)))], wn=float(rng.uniform(600.0, 1600.0)),
+                        steps=max(1, int(round(rng.uniform(5000, 20000) * step_scale))),
+                        gap=float(rng.exponential(1.0))))
+    return out
+
+
+_LATTICE_CACHE = {}
+
+
+def serving_case(req):
+    """(spec, medium) of one serving request: 20 nm cells, 10-cell CPML, a slab
+    of the request's material over the middle sixth of z, a z-dipole at the
+    source wavenumber 5 cells under it, the monitor 3 cells above it.  The
+    Lorentz material uses the ADE where the reference allows it (Re eps < 20)
+    and the lossy-dielectric form elsewhere."""
+    nx, ny, nz = req["shape"]
+    f = req["wn"] * WN_TO_HZ
+    g = _grid((nx, ny, nz), 20e-9, req["steps"], 10, f)
+    s0 = nz * 5 // 12
+    s1 = nz * 7 // 12
+    med = np.zeros((nx, ny, nz), np.int32)
+    med[:, :, s0:s1] = 4
+    kind = req["kind"]
+    if kind == "lorentz":
+        mat = material_cfg0(f)
+        # the reference's Drude-fit ADE needs Re eps < EPS_INF = 20
+        # (fdtd.c:2478-2485); nearer the 1370 cm^-1 pole the slab runs as a
+        # lossy dielectric (eps_im loss term), as a reference user would set it
+        if mat["eps_re"][0] >= 20.0:
+            mat = dict(mat, use_ade=0)
+    else:
+        if kind not in _LATTICE_CACHE:
+            _LATTICE_CACHE[kind] = material_cfg1() if kind == "lattice4" else material_cfg2()
+        mat = _LATTICE_CACHE[kind]
+    src = [_source("dipole", (nx // 2, ny // 2, s0 - 5), "z", f, tsteps=req["steps"], dt=g["dt"])]
+    spec = dict(name="req%d/%s/%dx%dx%d" % (req["id"], kind, nx, ny, nz), grid=g, materials=builtin_table(f) + [mat],
+                sources=src, monitor=dict(x=nx // 2, y=ny // 2, z=s1 + 3, atom=0, dir=2))
+    return spec, med
+
 
 def with_steps(spec, steps):
     spec = dict(spec)
