@@ -5,7 +5,8 @@
 //   bwprobe [GiB per buffer] [iterations] [variant|all]
 // variants: gs (grid-stride 16 B), u4 (one-shot, 4 x 16 B per thread, loads
 // first), nt (u4 with non-temporal loads/stores), rd (read-only reduction),
-// wr (write-only fill).
+// wr (write-only fill), ip (two buffers updated in place, non-temporal: two
+// read and two write streams at the same addresses, the lattice state pattern).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -33,6 +34,27 @@ __global__ __launch_bounds__(256) void copy_u4(const u32x4 *__restrict__ a, u32x
         if (q < n) {
             if (NT) __builtin_nontemporal_store(v[u], b + q);
             else b[q] = v[u];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void inplace2(u32x4 *__restrict__ a, u32x4 *__restrict__ b, size_t n) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    u32x4 va[4], vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t q = base + u * 256;
+        if (q < n) {
+            va[u] = __builtin_nontemporal_load(a + q);
+            vb[u] = __builtin_nontemporal_load(b + q);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t q = base + u * 256;
+        if (q < n) {
+            __builtin_nontemporal_store(va[u] ^ vb[u], a + q);
+            __builtin_nontemporal_store(vb[u] + 1u, b + q);
         }
     }
 }
@@ -83,9 +105,9 @@ int main(int argc, char **argv) {
     hipEvent_t e0, e1;
     CK(hipEventCreate(&e0));
     CK(hipEventCreate(&e1));
-    const char *names[] = {"gs", "u4", "nt", "rd", "wr"};
+    const char *names[] = {"gs", "u4", "nt", "rd", "wr", "ip"};
     const unsigned one_shot = (unsigned)((n + 1023) / 1024);
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < 6; ++v) {
         if (strcmp(which, "all") && strcmp(which, names[v])) continue;
         for (int it = -1; it < iters; ++it) {
             if (it == 0) CK(hipEventRecord(e0, 0));
@@ -95,6 +117,7 @@ int main(int argc, char **argv) {
                 case 2: hipLaunchKernelGGL(copy_u4<true>, dim3(one_shot), dim3(256), 0, 0, (const u32x4 *)a, (u32x4 *)b, n); break;
                 case 3: hipLaunchKernelGGL(read_u4, dim3(one_shot), dim3(256), 0, 0, a, b, n); break;
                 case 4: hipLaunchKernelGGL(write_u4, dim3(one_shot), dim3(256), 0, 0, b, n); break;
+                case 5: hipLaunchKernelGGL(inplace2, dim3(one_shot), dim3(256), 0, 0, (u32x4 *)a, (u32x4 *)b, n); break;
             }
             if (it == -1) CK(hipDeviceSynchronize());
         }
@@ -103,7 +126,7 @@ int main(int argc, char **argv) {
         float ms = 0;
         CK(hipEventElapsedTime(&ms, e0, e1));
         const double per = ms / iters;
-        const double moved = (v == 3 || v == 4) ? (double)bytes : 2.0 * bytes;
+        const double moved = (v == 3 || v == 4) ? (double)bytes : v == 5 ? 4.0 * bytes : 2.0 * bytes;
         printf("{\"variant\": \"%s\", \"bytes_per_buffer\": %zu, \"moved_bytes\": %.0f, \"ms\": %.4f, \"gbps\": %.1f}\n",
                names[v], bytes, moved, per, moved / (per * 1e-3) / 1e9);
     }

@@ -11,17 +11,16 @@
 //   a = QeZM^T E_cr - DMM r^n;  r += v dt + a dt^2/2;  v += a dt  (3505-3528)
 //   E = R^T E_cr                                                  (3489-3503)
 //
-// Lattice state layout (IonLayout): tiles of 64 slots; the plain form keeps
-// r[tile][dof][64], the precision form the MFMA operand orders, so every
-// wavefront access is 512 contiguous bytes streamed non-temporally.
+// Lattice state layout (IonLayout), both forms: tiles of 64 slots,
+// r[tile][dof][64], so a lane owns a slot and every wavefront access is 512
+// contiguous bytes streamed non-temporally.
 //
 // Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
 // DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
-// Precision form (FP32): one wavefront per 64-slot tile; [DMM; QeZV] r^n is
-// one v_mfma_f32_16x16x4f32 contraction per 16-slot group (A = the stacked
-// 3N+3 x 3N matrix, pre-arranged in lane order on the host; B = r^n rounded
-// to FP32), accumulated in FP32; P_ion falls out of the same product.  The
-// r, v integration stays FP64.
+// Precision form (FP32): the same slot-per-lane streams; -DMM r^n runs on FP32
+// MFMA (v_mfma_f32_16x16x4f32) after an LDS transpose of r^n rounded to FP32,
+// accumulated in FP32; P_ion and QeZM^T E_cr on the VALU in FP32.  The r, v
+// integration stays FP64.
 #include "phpfdtd_internal.h"
 
 namespace phpfdtd {
@@ -148,161 +147,161 @@ __device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const do
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Precision form: one wavefront per 64-slot tile, on FP32 MFMA.  Operand
-// layouts of v_mfma_f32_16x16x4f32 (lane l, gfx950):
-//   A[m][k] in lane m + 16k,  B[k][n] in lane n + 16k,
-//   D[m][n] in lane n + 16 (m / 4), element m % 4.
-// r and v are stored in B-operand order (IonLayout, mfma = 1), so lane l of
-// k-tile kt of group g owns dof 4 kt + l / 16 of slot 16 g + l % 16 and every
-// global access of the kernel is one 512-byte row.  Per 16-slot group g:
-//   1. acc = [-DMM; 0; QeZV] . r^n over the k-tiles (B = r^n rounded to
-//      FP32); the QeZV rows at PR = roundup(3N, 4) give P_ion in elements
-//      0..2 of lane group (PR % 16) / 4 of m-tile PR / 16.
-//   2. lane 16g + n (which prefetched D of slot 16g + n) forms E_cr of its
-//      slot; E_cr is spread as one more k-tile (E component k in lane
-//      n + 16k) and acc += QeZM^T . E_cr, one MFMA per m-tile, so
-//      acc = QeZM^T E_cr - DMM r^n = a (accumulator order).
-//   3. a is transposed to B order through a 17-float-pitch LDS tile of the
-//      wavefront, and each lane integrates its own dofs in FP64 from the r^n
-//      it still holds and the v it prefetched.
-// The groups are software-pipelined: r^n of group g+1 and v of group g are
-// requested before group g's MFMA chain.
-// aop[(mt*nk + kt)*64 + l] and qop[mt*64 + l] hold the A operands in lane order.
-template <int NA>
+// Precision form: lane l of a wavefront owns slot 64 tile + l and all 3N dofs
+// of its r^n and v^n - the memory pattern of the plain form (r[tile][dof][64],
+// every access one 512-byte row, all of r^n requested up front).  Only the
+// contraction -DMM . r^n runs on FP32 MFMA (v_mfma_f32_16x16x4f32; lane l on
+// gfx950: A[m][k] in lane m + 16k, B[k][n] in lane n + 16k, D[m][n] in lane
+// n + 16 (m / 4), element m % 4):
+//   1. r^n is rounded to FP32 into LDS rows (dof q, 64 slots) of the
+//      wavefront; P_ion = QeZV r^n on the VALU in FP32;
+//   2. E_cr and the lab-frame E of the lane's slot, stored by the whole
+//      wavefront (fdtd.c:3404-3503);
+//   3. per 16-slot group g: acc_g = -DMM . r^n over the k-tiles, B read from
+//      the LDS rows, A (-DMM in lane order) from an LDS table staged once per
+//      work-group;
+//   4. acc_g goes back to the LDS rows (dof m, slot 16g + n) and each lane
+//      reads its own dofs: a = acc + QeZM^T E_cr (VALU, FP32), then the FP64
+//      integration r += v dt + a dt^2/2, v += a dt (fdtd.c:3505-3528).
+// QeZV / QeZM are read through the scalar cache from qop = [QeZV | QeZM]
+// (FP32).  Padding slots past nslot hold r = v = 0 and have no cell
+// (E_cr = 0), so their acceleration is exactly 0 and their unguarded stores
+// keep them at 0.  NA = kMaxAtom is the generic instantiation (runtime atom
+// count, operand table read from global memory, one wavefront per group).
+// LDS row pitches (floats): B operands are read as 4 rows x 16 consecutive
+// slots per half-wavefront, conflict-free for a pitch of 16 mod 32; the
+// accumulator is written as rows 4 apart, conflict-free for 4 x pitch of
+// 16 mod 32.
+constexpr int kRowB = 80, kRowA = 68;
+
+// Per-wavefront LDS floats for N atoms: the r^n rows, later reused for the
+// acceleration rows.
+__host__ __device__ constexpr int mfma_wave_lds(int na) {
+    return (3 * na + 3) / 4 * 4 * kRowB > (3 * na + 15) / 16 * 16 * kRowA ? (3 * na + 3) / 4 * 4 * kRowB
+                                                                         : (3 * na + 15) / 16 * 16 * kRowA;
+}
+
+template <int NA, int WPB>
 __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const float *__restrict__ aop,
                                                const float *__restrict__ qop, double *__restrict__ rr,
                                                double *__restrict__ vv, const int64_t blk) {
-    constexpr int NMX = ((3 * NA + 3) / 4 * 4 + 3 + 15) / 16;
-    constexpr int NKX = (3 * NA + 3) / 4;
-    constexpr int NAX = (3 * NA + 15) / 16;
-    const int lane = threadIdx.x & 63;
-    const int64_t tile = blk * 4 + (threadIdx.x >> 6);
-    const int64_t ntile = (a.nslot + 63) >> 6;
-    if (tile >= ntile) return;  // wave-uniform
-    const IonLayout L = a.lay;
-    // exact instantiations (natom == NA) use compile-time tile counts, so every
-    // operand array is fully unrolled into registers; the generic one (NA =
-    // kMaxAtom) runs to the material's own counts
     constexpr bool exact = NA != kMaxAtom;
-    const int N3 = exact ? 3 * NA : L.n3;
-    const int nk = exact ? NKX : L.nk, nma = exact ? NAX : L.nma;
-    const int nm = exact ? NMX : ((N3 + 3) / 4 * 4 + 3 + 15) / 16;
-    const int PR = (N3 + 3) & ~3;
-    const int pm = PR >> 4, pg = (PR & 15) >> 2;
+    constexpr int N3X = 3 * NA;
+    constexpr int NKX = (N3X + 3) / 4, NAX = (N3X + 15) / 16;
+    constexpr int WL = mfma_wave_lds(NA);
+    const int natom = exact ? NA : a.natom;
+    const int N3 = 3 * natom;
+    const int NK = exact ? NKX : (N3 + 3) / 4, NMA = exact ? NAX : (N3 + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 4, col = lane & 15;
-    const int64_t base = tile * L.tile_r();
+    const int64_t tile = blk * WPB + wave;
+    const int64_t ntile = (a.nslot + 63) >> 6;
     extern __shared__ float lds_a[];
-    float *at = lds_a + (threadIdx.x >> 6) * (NKX * 4 * 17);  // this wavefront's a[dof][17]
-
-    float A[NMX][NKX], AQ[NAX];
+    const float *atab = aop;
+    if (exact) {
+        float *t = lds_a + WPB * WL;
+        for (int w = threadIdx.x; w < NAX * NKX * 64; w += 64 * WPB) t[w] = aop[w];
+        __syncthreads();
+        atab = t;
+    }
+    if (tile >= ntile) return;  // wave-uniform
+    float *buf = lds_a + wave * WL;
+    const int64_t tb = tile * 64 * (int64_t)N3 + lane;  // dof q of this lane's slot at tb + 64 q
+    double r[N3X];
 #pragma unroll
-    for (int mt = 0; mt < NMX; ++mt)
-#pragma unroll
-        for (int kt = 0; kt < NKX; ++kt) A[mt][kt] = (mt < nm && kt < nk) ? aop[(mt * nk + kt) * 64 + lane] : 0.f;
-#pragma unroll
-    for (int mt = 0; mt < NAX; ++mt) AQ[mt] = mt < nma ? qop[mt * 64 + lane] : 0.f;
-
-    // lane l prefetches D (and E^n for ADE) of slot 64 tile + l
-    const int64_t my_slot = tile * 64 + lane;
-    const bool my_live = my_slot < a.nslot;
-    size_t my_cell = 0;
+    for (int q = 0; q < N3X; ++q)
+        if (q < N3) r[q] = ldnt(rr + tb + q * 64);
+    const int64_t slot = tile * 64 + lane;
+    const bool live = slot < a.nslot;
+    size_t cell = 0;
     float Dl[3] = {0.f, 0.f, 0.f}, Eold[3] = {0.f, 0.f, 0.f};
-    if (my_live) {
-        my_cell = a.cells[my_slot];
-        Dl[0] = a.Dx[my_cell];
-        Dl[1] = a.Dy[my_cell];
-        Dl[2] = a.Dz[my_cell];
+    if (live) {
+        cell = a.cells[slot];
+        Dl[0] = a.Dx[cell];
+        Dl[1] = a.Dy[cell];
+        Dl[2] = a.Dz[cell];
         if (a.ade) {
-            Eold[0] = a.Ex[my_cell];
-            Eold[1] = a.Ey[my_cell];
-            Eold[2] = a.Ez[my_cell];
+            Eold[0] = a.Ex[cell];
+            Eold[1] = a.Ey[cell];
+            Eold[2] = a.Ez[cell];
         }
     }
-    const double dt = a.dt, hdt2 = a.half_dt2;
-    double rnext[NKX];
+    // 1. r^n in FP32 as LDS rows (rows 3N .. 4 nk of the last k-tile zero), P_ion
+    float P[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < NKX; ++kt) rnext[kt] = kt < nk ? ldnt(rr + base + (int64_t)kt * 64 + lane) : 0.0;
-#pragma unroll 1
+    for (int q = 0; q < N3X; ++q) {
+        if (q >= N3) break;
+        const float rf = (float)r[q];
+        buf[q * kRowB + lane] = rf;
+        const int at = q / 3, d = q % 3;
+        P[0] += sld(qop, at * 9 + 0 + d) * rf;
+        P[1] += sld(qop, at * 9 + 3 + d) * rf;
+        P[2] += sld(qop, at * 9 + 6 + d) * rf;
+    }
+    for (int q = N3; q < 4 * NK; ++q) buf[q * kRowB + lane] = 0.f;
+    // 2. E of this lane's slot
+    float Ec[3] = {0.f, 0.f, 0.f};
+    if (live) {
+        float Dc[3];
+        rot(a.R, Dl, Dc);
+        float Dp[3] = {Dc[0] - P[0], Dc[1] - P[1], Dc[2] - P[2]};
+        if (a.ade) {
+            float Eo[3], Pn[3];
+            rot(a.R, Eold, Eo);
+            ade_step(a, slot, Eo, Pn);
+            for (int e = 0; e < 3; ++e) Dp[e] -= Pn[e];
+        }
+        rot(a.A, Dp, Ec);
+        float El[3];
+        rot_t(a.R, Ec, El);
+        a.Ex[cell] = El[0];
+        a.Ey[cell] = El[1];
+        a.Ez[cell] = El[2];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // 3. -DMM . r^n per 16-slot group on MFMA
+    f32x4 acc[4][NAX];
+#pragma unroll
     for (int g = 0; g < 4; ++g) {
-        double rcur[NKX], vcur[NKX];
+#pragma unroll
+        for (int mt = 0; mt < NAX; ++mt) acc[g][mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < NKX; ++kt) {
-            rcur[kt] = rnext[kt];
-            vcur[kt] = kt < nk ? ldnt(vv + base + ((int64_t)g * nk + kt) * 64 + lane) : 0.0;
+            if (kt >= NK) break;
+            const float b = buf[(4 * kt + grp) * kRowB + 16 * g + col];
+#pragma unroll
+            for (int mt = 0; mt < NAX; ++mt)
+                if (mt < NMA)
+                    acc[g][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(atab[(mt * NK + kt) * 64 + lane], b, acc[g][mt],
+                                                                     0, 0, 0);
         }
-        if (g < 3) {
+    }
+    __builtin_amdgcn_wave_barrier();
+    // 4. acceleration rows back to LDS (dof m, slot 16g + col), then per dof
 #pragma unroll
-            for (int kt = 0; kt < NKX; ++kt)
-                rnext[kt] = kt < nk ? ldnt(rr + base + ((int64_t)(g + 1) * nk + kt) * 64 + lane) : 0.0;
-        }
-        f32x4 acc[NMX];
-#pragma unroll
-        for (int mt = 0; mt < NMX; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kt = 0; kt < NKX; ++kt) {
-            if (kt >= nk) break;
-            const float b = (float)rcur[kt];
-#pragma unroll
-            for (int mt = 0; mt < NMX; ++mt)
-                if (mt < nm) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][kt], b, acc[mt], 0, 0, 0);
-        }
-        // P_ion of slot 16g + col to lane 16g + col
-        float p0 = 0.f, p1 = 0.f, p2 = 0.f;
-#pragma unroll
-        for (int mt = 0; mt < NMX; ++mt)
-            if (mt == pm) { p0 = acc[mt][0]; p1 = acc[mt][1]; p2 = acc[mt][2]; }
-        const int psrc = pg * 16 + col;
-        p0 = __shfl(p0, psrc, 64);
-        p1 = __shfl(p1, psrc, 64);
-        p2 = __shfl(p2, psrc, 64);
-        float Ec[3] = {0.f, 0.f, 0.f};
-        if (grp == g && my_live) {
-            float Dc[3];
-            rot(a.R, Dl, Dc);
-            float Dp[3] = {Dc[0] - p0, Dc[1] - p1, Dc[2] - p2};
-            if (a.ade) {
-                float Eo[3], Pn[3];
-                rot(a.R, Eold, Eo);
-                ade_step(a, my_slot, Eo, Pn);
-                for (int e = 0; e < 3; ++e) Dp[e] -= Pn[e];
-            }
-            rot(a.A, Dp, Ec);
-            float El[3];
-            rot_t(a.R, Ec, El);
-            a.Ex[my_cell] = El[0];
-            a.Ey[my_cell] = El[1];
-            a.Ez[my_cell] = El[2];
-        }
-        // E_cr of slot 16g + col as the B operand: component k in lane col + 16k
-        const int esrc = 16 * g + col;
-        const float e0 = __shfl(Ec[0], esrc, 64), e1 = __shfl(Ec[1], esrc, 64), e2 = __shfl(Ec[2], esrc, 64);
-        const float eb = grp == 0 ? e0 : grp == 1 ? e1 : grp == 2 ? e2 : 0.f;
-#pragma unroll
-        for (int mt = 0; mt < NAX; ++mt)
-            if (mt < nma) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(AQ[mt], eb, acc[mt], 0, 0, 0);
-        // a from accumulator order to B order through LDS (same wavefront only)
-        __builtin_amdgcn_wave_barrier();
+    for (int g = 0; g < 4; ++g)
 #pragma unroll
         for (int mt = 0; mt < NAX; ++mt) {
-            if (mt >= nma) break;
+            if (mt >= NMA) break;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int m = 16 * mt + 4 * grp + i;
-                if (m < N3) at[m * 17 + col] = acc[mt][i];
+                if (m < N3) buf[m * kRowA + 16 * g + col] = acc[g][mt][i];
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        const bool live = tile * 64 + 16 * g + col < a.nslot;
+    __builtin_amdgcn_wave_barrier();
+    const double dt = a.dt, hdt2 = a.half_dt2;
+    const float *qm = qop + 9 * natom;
 #pragma unroll
-        for (int kt = 0; kt < NKX; ++kt) {
-            if (kt >= nk) break;
-            const int dof = 4 * kt + grp;
-            if (dof >= N3 || !live) continue;
-            const double am = (double)at[dof * 17 + col];
-            const int64_t idx = base + ((int64_t)g * nk + kt) * 64 + lane;
-            stnt(rcur[kt] + (vcur[kt] * dt + am * hdt2), rr + idx);
-            stnt(vcur[kt] + am * dt, vv + idx);
-        }
+    for (int q = 0; q < N3X; ++q) {
+        if (q >= N3) break;
+        const int at = q / 3, d = q % 3;
+        const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] + sld(qm, at * 9 + 3 + d) * Ec[1] +
+                         sld(qm, at * 9 + 6 + d) * Ec[2];
+        const double am = (double)af;
+        const double vq = ldnt(vv + tb + q * 64);
+        stnt(r[q] + (vq * dt + am * hdt2), rr + tb + q * 64);
+        stnt(vq + am * dt, vv + tb + q * 64);
     }
 }
 
@@ -320,10 +319,10 @@ __global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const 
 }
 
 template <int NA>
-__global__ __launch_bounds__(256) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
                                                        const float *__restrict__ qop, double *__restrict__ rr,
                                                        double *__restrict__ vv) {
-    ion_mfma_block<NA>(a, aop, qop, rr, vv, blockIdx.x);
+    ion_mfma_block<NA, NA == kMaxAtom ? 1 : 4>(a, aop, qop, rr, vv, blockIdx.x);
 }
 
 // Batched launches over the slot work of several domains: work-group g is
@@ -343,10 +342,10 @@ __global__ __launch_bounds__(256) void ion_update_f64_batch(BatchLaunch<SlotArgs
 }
 
 template <int NA>
-__global__ __launch_bounds__(256) void ion_update_mfma_batch(BatchLaunch<SlotArgs<float>> b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ion_update_mfma_batch(BatchLaunch<SlotArgs<float>> b) {
     const int rid = (int)sld(b.map, blockIdx.x);
     const SlotArgs<float> a = batch_args(b, rid);
-    ion_mfma_block<NA>(a, a.aop, a.qop, a.r, a.v, blockIdx.x - sld(b.start, rid));
+    ion_mfma_block<NA, NA == kMaxAtom ? 1 : 4>(a, a.aop, a.qop, a.r, a.v, blockIdx.x - sld(b.start, rid));
 }
 
 template <typename real>
@@ -382,18 +381,29 @@ void launch_ion<double>(const SlotArgs<double> &a, hipStream_t st) {
 #undef PHPFDTD_ION_CASE
 }
 
+// Dynamic LDS of the precision-form lattice kernel: per-wavefront transpose
+// rows, plus the operand table staged per work-group (exact instantiations).
+static size_t mfma_lds_bytes(int na) {
+    if (na == kMaxAtom) return (size_t)mfma_wave_lds(na) * 4;
+    return ((size_t)4 * mfma_wave_lds(na) + (size_t)mfma_nma(na) * mfma_nk(na) * 64) * 4;
+}
+
+// Threads per work-group of the precision-form lattice kernel.
+static int mfma_threads(int na) { return na == kMaxAtom ? 64 : 256; }
+
 template <>
 void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
     const int64_t ntile = (a.nslot + 63) / 64;
-    const unsigned nb = (unsigned)((ntile + 3) / 4);
-    if (!nb) return;
+    if (!ntile) return;
     // the host lays out aop/qop for the instantiation mfma_inst(natom)
     switch (mfma_inst(a.natom)) {
-#define PHPFDTD_MFMA_CASE(NN)                                                                                \
-    case NN:                                                                                                 \
-        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3(nb), dim3(256), (size_t)4 * mfma_nk(NN) * 4 * 17 * 4, st, a, \
-                           a.aop, a.qop, a.r, a.v);                                                          \
-        return;
+#define PHPFDTD_MFMA_CASE(NN)                                                                  \
+    case NN: {                                                                                 \
+        const int wpb = mfma_threads(NN) / 64;                                                 \
+        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3((unsigned)((ntile + wpb - 1) / wpb)),     \
+                           dim3(mfma_threads(NN)), mfma_lds_bytes(NN), st, a, a.aop, a.qop, a.r, a.v); \
+        return;                                                                                \
+    }
         PHPFDTD_MFMA_CASE(1)
         PHPFDTD_MFMA_CASE(2)
         PHPFDTD_MFMA_CASE(3)
@@ -410,7 +420,10 @@ void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
 // Work-groups of one domain's slot kernel (the unit of a batched launch).
 template <typename real>
 int64_t slot_blocks(const SlotArgs<real> &a) {
-    if (a.natom > 0 && sizeof(real) == 4) return ((a.nslot + 63) / 64 + 3) / 4;
+    if (a.natom > 0 && sizeof(real) == 4) {
+        const int wpb = mfma_threads(mfma_inst(a.natom)) / 64;
+        return ((a.nslot + 63) / 64 + wpb - 1) / wpb;
+    }
     return (a.nslot + 255) / 256;
 }
 
@@ -457,7 +470,8 @@ void launch_slot_batch<float>(const BatchLaunch<SlotArgs<float>> &b, int kind, h
     switch (kind) {
         case 0: hipLaunchKernelGGL(ade_update_batch<float>, g, blk, 0, st, b); return;
 #define PHPFDTD_MFMA_CASE(NN) \
-    case NN: hipLaunchKernelGGL(ion_update_mfma_batch<NN>, g, blk, (size_t)4 * mfma_nk(NN) * 4 * 17 * 4, st, b); return;
+    case NN:                  \
+        hipLaunchKernelGGL(ion_update_mfma_batch<NN>, g, dim3(mfma_threads(NN)), mfma_lds_bytes(NN), st, b); return;
         PHPFDTD_MFMA_CASE(1)
         PHPFDTD_MFMA_CASE(2)
         PHPFDTD_MFMA_CASE(3)

@@ -73,7 +73,7 @@ struct Special {
     int id = 0;
     MatPre pre{};
     std::vector<uint32_t> cells;  // x-fastest field indices, ascending
-    IonLayout lay{};              // lattice state layout (plain or MFMA operand order)
+    IonLayout lay{};              // lattice state layout r[tile][dof][64]
     std::vector<double> qezv, qezm, dmm;
     uint32_t *d_cells = nullptr;
     void *d_P = nullptr, *d_Po = nullptr;
@@ -230,9 +230,11 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
         l.slots += n * 4;
         if (ade[m]) l.slots += 2 * 3 * n * rs;
         if (natom[m]) {
-            const IonLayout il = ion_layout(natom[m], mfma);
+            const IonLayout il = ion_layout(natom[m]);
             l.slots += (n + 63) / 64 * (size_t)(il.tile_r() + il.tile_v()) * 8 + 9 * natom[m] * natom[m] * 8 +
-                       18 * natom[m] * 8 + 7 * 24 * 64 * 4 + 7 * 64 * 4;
+                       18 * natom[m] * 8;
+            if (mfma)
+                l.slots += ((size_t)mfma_nma(natom[m]) * mfma_nk(natom[m]) * 64 + 18 * natom[m]) * sizeof(float);
         }
     }
     l.ring = (size_t)g.tsteps * 12 * 8;
@@ -262,30 +264,25 @@ static int count_medium(const phpfdtd_grid &g, const int32_t *medium, int nmat, 
     return 0;
 }
 
-// MFMA operand tables of the precision-form lattice kernel, in lane order
-// (A[m][k] of m-tile mt, k-tile kt in lane m%16 + 16 (k%4)):
-//   aop: [-DMM; 0; QeZV; 0], the 3 QeZV rows at row roundup(3N, 4);
-//   qop: QeZM^T, column k = E component (k < 3), for the acceleration rows.
+// Operand tables of the precision-form lattice kernel:
+//   aop: -DMM as v_mfma_f32_16x16x4f32 A operands, A[m][k] of m-tile mt and
+//        k-tile kt in lane m % 16 + 16 (k % 4) at aop[(mt*nk + kt)*64 + lane];
+//   qop: [QeZV | QeZM] in FP32, 9N each, [atom][E component][direction].
 static void mfma_tables(const Special &sp, std::vector<float> *aop, std::vector<float> *qop) {
     const int N = sp.pre.natom, N3 = 3 * N;
-    const int NM = mfma_nm(N), NK = mfma_nk(N), NA = mfma_nma(N);
-    const int PR = (N3 + 3) / 4 * 4;
-    auto A = [&](int m, int k) -> double {
-        if (k >= N3) return 0.0;
-        if (m < N3) return -sp.dmm[(size_t)m * N3 + k];
-        if (m >= PR && m < PR + 3) return sp.qezv[(k / 3) * 9 + (m - PR) * 3 + (k % 3)];
-        return 0.0;
-    };
-    aop->assign((size_t)NM * NK * 64, 0.f);
-    for (int mt = 0; mt < NM; ++mt)
-        for (int kt = 0; kt < NK; ++kt)
-            for (int l = 0; l < 64; ++l) (*aop)[((size_t)mt * NK + kt) * 64 + l] = (float)A(16 * mt + l % 16, 4 * kt + l / 16);
-    qop->assign((size_t)NA * 64, 0.f);
+    const int NK = mfma_nk(N), NA = mfma_nma(N);
+    aop->assign((size_t)NA * NK * 64, 0.f);
     for (int mt = 0; mt < NA; ++mt)
-        for (int l = 0; l < 64; ++l) {
-            const int m = 16 * mt + l % 16, k = l / 16;
-            if (m < N3 && k < 3) (*qop)[(size_t)mt * 64 + l] = (float)sp.qezm[(m / 3) * 9 + k * 3 + (m % 3)];
-        }
+        for (int kt = 0; kt < NK; ++kt)
+            for (int l = 0; l < 64; ++l) {
+                const int m = 16 * mt + l % 16, k = 4 * kt + l / 16;
+                if (m < N3 && k < N3) (*aop)[((size_t)mt * NK + kt) * 64 + l] = (float)-sp.dmm[(size_t)m * N3 + k];
+            }
+    qop->assign((size_t)18 * N, 0.f);
+    for (int q = 0; q < 9 * N; ++q) {
+        (*qop)[q] = (float)sp.qezv[q];
+        (*qop)[9 * N + q] = (float)sp.qezm[q];
+    }
 }
 
 template <typename real> static YeeArgs<real> &args_of(phpfdtd_ctx *x);
@@ -626,7 +623,7 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
             if ((rc = x->alloc(&sp.d_Po, 3 * n * x->rs))) return rc;
         }
         if (sp.pre.natom > 0) {
-            sp.lay = ion_layout(sp.pre.natom, x->prec == PHPFDTD_FP32);
+            sp.lay = ion_layout(sp.pre.natom);
             const size_t ntile = (n + 63) / 64;  // 64-slot tiles
             if ((rc = x->alloc((void **)&sp.d_r, ntile * sp.lay.tile_r() * 8))) return rc;
             if ((rc = x->alloc((void **)&sp.d_v, ntile * sp.lay.tile_v() * 8))) return rc;

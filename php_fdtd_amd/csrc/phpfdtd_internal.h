@@ -11,7 +11,7 @@
 //       x slabs [nz][ny][2L], y slabs [nz][2L][nxp], z slabs [2L][ny][nxp].
 //   * ADE and lattice state: per-material slot arrays, slot s of material m
 //     is the s-th cell of m in x-fastest order; ADE P[3][slot]; lattice r, v
-//     in 64-slot tiles r[tile][dof][64] (ion_index).
+//     in 64-slot tiles r[tile][dof][64] (IonLayout).
 // FP64 host precompute (CPML b/c/kappa, eps^-1, ADE coefficients, rotations,
 // source values) is cast to the working type only when uploaded.
 #pragma once
@@ -77,15 +77,13 @@ struct YeeArgs {
     const int64_t *step;
 };
 
-// MFMA tiling of the lattice contraction for N atoms: k-tiles of 4 dofs over
-// the 3N dofs, m-tiles of 16 rows over [DMM; 0; QeZV] (the 3 QeZV rows sit at
-// row roundup(3N, 4)), and m-tiles over the 3N acceleration rows alone.
+// MFMA tiling of the lattice contraction -DMM . r for N atoms: k-tiles of 4
+// dofs and m-tiles of 16 rows over the 3N dofs.
 __host__ __device__ inline int mfma_nk(int na) { return (3 * na + 3) / 4; }
-__host__ __device__ inline int mfma_nm(int na) { return ((3 * na + 3) / 4 * 4 + 3 + 15) / 16; }
 __host__ __device__ inline int mfma_nma(int na) { return (3 * na + 15) / 16; }
 
-// natom of the MFMA kernel instantiation (register arrays sized for it; loops
-// run to the actual tile counts of the material).
+// natom of the lattice kernel instantiation (register arrays sized for it;
+// the generic one runs to the material's own counts).
 inline int mfma_inst(int natom) {
     switch (natom) {
         case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return natom;
@@ -93,47 +91,29 @@ inline int mfma_inst(int natom) {
     }
 }
 
-// Lattice state layout of one material (r and v, FP64, tiles of 64 slots).
-//  plain form (mfma = 0): r[tile][dof][64] and v likewise - a thread owns a
-//    slot and reads 64 consecutive slots of one dof per wavefront access.
-//  precision form (mfma = 1): r and v in MFMA B-operand order
-//    [tile][g][kt][lane], lane = n + 16 (dof % 4), dof = 4 kt + lane / 16,
-//    slot = 16 g + n.
-//  Every wavefront access of the lattice kernels is then 512 contiguous bytes.
+// Lattice state layout of one material (r and v, FP64, both forms): tiles of
+// 64 slots, r[tile][dof][64] - a lane owns a slot and every wavefront access
+// is one 512-byte row of 64 slots of one dof.
 struct IonLayout {
-    int mfma, n3, nk, nma;
-    __host__ __device__ int64_t tile_r() const { return mfma ? 256LL * nk : 64LL * n3; }
+    int n3;
+    __host__ __device__ int64_t tile_r() const { return 64LL * n3; }
     __host__ __device__ int64_t tile_v() const { return tile_r(); }
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
-        const int64_t t = s >> 6;
-        const int sl = (int)(s & 63);
-        if (!mfma) return t * tile_r() + (int64_t)q * 64 + sl;
-        const int g = sl >> 4, n = sl & 15;
-        return t * tile_r() + ((int64_t)g * nk + (q >> 2)) * 64 + n + 16 * (q & 3);
+        return (s >> 6) * tile_r() + (int64_t)q * 64 + (s & 63);
     }
     __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
     // slot and dof of r element idx; false for padding
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
         const int64_t t = idx / tile_r(), rem = idx % tile_r();
-        if (!mfma) {
-            *q = (int)(rem / 64);
-            *s = t * 64 + rem % 64;
-        } else {
-            const int gk = (int)(rem / 64), lane = (int)(rem % 64);
-            const int g = gk / nk, kt = gk % nk;
-            *q = 4 * kt + lane / 16;
-            *s = t * 64 + 16 * g + lane % 16;
-        }
+        *q = (int)(rem / 64);
+        *s = t * 64 + rem % 64;
         return *q < n3 && *s < nslot;
     }
 };
 
-inline IonLayout ion_layout(int natom, bool mfma) {
+inline IonLayout ion_layout(int natom) {
     IonLayout l;
-    l.mfma = mfma ? 1 : 0;
     l.n3 = 3 * natom;
-    l.nk = mfma_nk(natom);
-    l.nma = mfma_nma(natom);
     return l;
 }
 
@@ -158,8 +138,8 @@ struct SlotArgs {
     const double *qezv;      // [N][3][3]
     const double *qezm;      // [N][3][3]
     const double *dmm;       // [3N][3N]
-    const float *aop;        // precision form: [-DMM; 0; QeZV] MFMA A operands in lane order
-    const float *qop;        // precision form: QeZM^T MFMA A operands in lane order
+    const float *aop;        // precision form: -DMM as MFMA A operands in lane order
+    const float *qop;        // precision form: [QeZV | QeZM] in FP32 (9N each)
     double dt, half_dt2;
 };
 

@@ -14,7 +14,7 @@ import pytest
 
 import golden_cases
 from oracle import orc
-from php_fdtd_amd import _abi
+from php_fdtd_amd import _abi, workloads
 from php_fdtd_amd.sim import Simulation
 
 pytestmark = pytest.mark.gpu
@@ -160,6 +160,46 @@ def test_random_state_vs_oracle(name):
     if v[2] > 0:
         assert [d.r_i, d.r_j, d.r_k, d.r_atom, d.r_dir] == idx
     sim.close()
+
+
+def small_lattice_case(natom):
+    """16x16x12 grid, 3-cell CPML, a rotated slab of a synthetic natom-atom
+    lattice (atom counts outside the kernels' exact instantiations take the
+    generic path)."""
+    f = 1000.0 * workloads.WN_TO_HZ
+    g = workloads._grid((16, 16, 12), 20e-9, 40, 3, f)
+    lat = workloads.lattice(natom, [20.0 + 3 * a for a in range(natom)], 40.0, 500.0, 1500.0, 2.0, 100 + natom)
+    mat = workloads.vacuum()
+    mat.update(eps_re=workloads.diag(3.0, 3.5, 2.5), anglez=0.3, natom=natom, qezv=lat["qezv"], qezm=lat["qezm"],
+               dmm=lat["dmm"])
+    med = np.zeros((16, 16, 12), np.int32)
+    med[:, :, 4:8] = 4
+    src = [workloads._source("dipole", (8, 8, 2), "z", f, tsteps=40, dt=g["dt"])]
+    spec = dict(name="lattice%d" % natom, grid=g, materials=workloads.builtin_table(f) + [mat], sources=src)
+    return spec, med
+
+
+@pytest.mark.parametrize("natom", [1, 3, 5])
+def test_lattice_atom_counts(natom):
+    spec, med = small_lattice_case(natom)
+    print("lattice natom", natom, flush=True)
+    sims = {p: Simulation(spec, med, p) for p in (_abi.FP64, _abi.FP32)}
+    o = orc.Oracle(spec, med)
+    randomize(spec, med, 99, sims[_abi.FP64], o)
+    randomize(spec, med, 99, sims[_abi.FP32], orc.Oracle(spec, med))
+    for s in sims.values():
+        s.run(5)
+    o.step(5)
+    comps = list(range(9))
+    for comp in comps:
+        assert rel_l2(sims[_abi.FP64].field(comp), o.field(comp)) <= RANDOM_TOL, comp
+        assert rel_l2(sims[_abi.FP32].field(comp), o.field(comp)) <= FP32_TOL, comp
+    for kind in (0, 1):
+        ref = o.state(4, kind)
+        assert rel_l2(sims[_abi.FP64].state(4, kind), ref) <= RANDOM_TOL, kind
+        assert rel_l2(sims[_abi.FP32].state(4, kind), ref) <= FP32_TOL, kind
+    for s in sims.values():
+        s.close()
 
 
 def test_graph_matches_stream_launches(monkeypatch):
