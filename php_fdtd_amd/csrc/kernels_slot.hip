@@ -11,9 +11,10 @@
 //   a = QeZM^T E_cr - DMM r^n;  r += v dt + a dt^2/2;  v += a dt  (3505-3528)
 //   E = R^T E_cr                                                  (3489-3503)
 //
-// Lattice state layout (IonLayout), both forms: tiles of 64 slots,
-// r[tile][dof][64], so a lane owns a slot and every wavefront access is 512
-// contiguous bytes streamed non-temporally.
+// Lattice state layout (IonLayout), both forms: tiles of 64 slots with the
+// dofs in pairs, r[tile][dof / 2][64][2], so a lane owns a slot, reads two
+// dofs as one 16-byte vector and every wavefront access is 1 KiB contiguous,
+// streamed non-temporally.
 //
 // Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
 // DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
@@ -47,6 +48,16 @@ __device__ __forceinline__ T ldnt(const T *p) {
 template <typename T>
 __device__ __forceinline__ void stnt(T v, T *p) {
     __builtin_nontemporal_store(v, p);
+}
+
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Dof pair p of the slot at lane offset `base` (IonLayout: pairs 128 doubles apart).
+__device__ __forceinline__ f64x2 ldpair(const double *base, int p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(base + (int64_t)p * 128));
+}
+__device__ __forceinline__ void stpair(f64x2 v, double *base, int p) {
+    __builtin_nontemporal_store(v, reinterpret_cast<f64x2 *>(base + (int64_t)p * 128));
 }
 
 // ADE recurrence in the crystal frame for one slot; returns P^{n+1}.
@@ -92,16 +103,21 @@ template <int NA>
 __device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const double *__restrict__ dmm,
                                              const double *__restrict__ qezv, const double *__restrict__ qezm,
                                              double *__restrict__ rr, double *__restrict__ vv, const int64_t s) {
-    constexpr int NMAX = NA > 0 ? 3 * NA : 3 * kMaxAtom;
+    constexpr int NMAX = NA > 0 ? (3 * NA + 1) & ~1 : 3 * kMaxAtom;  // even: whole dof pairs
     if (s >= a.nslot) return;
     const int N3 = NA > 0 ? 3 * NA : 3 * a.natom;
+    const int NP = (N3 + 1) / 2;
     const size_t c = a.cells[s];
-    const int64_t tb = a.lay.r_index(s, 0);  // dof q at tb + 64 q (plain layout, same for v)
+    const int64_t tb = a.lay.r_index(s, 0);  // dof pair p at tb + 128 p (same for v)
     const double Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
     double r[NMAX];
 #pragma unroll
-    for (int q = 0; q < NMAX; ++q)
-        if (q < N3) r[q] = ldnt(rr + tb + q * 64);
+    for (int p = 0; p < NMAX / 2; ++p)
+        if (p < NP) {
+            const f64x2 v2 = ldpair(rr + tb, p);
+            r[2 * p] = v2.x;
+            r[2 * p + 1] = v2.y;
+        }
     double Dc[3];
     rot(a.R, Dl, Dc);
     double P[3] = {0, 0, 0};
@@ -130,26 +146,34 @@ __device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const do
     a.Ez[c] = El[2];
     const double dt = a.dt, hdt2 = a.half_dt2;
 #pragma unroll
-    for (int q = 0; q < NMAX; ++q) {
-        if (q < N3) {
+    for (int p = 0; p < NMAX / 2; ++p) {
+        if (p >= NP) break;
+        f64x2 v2 = ldpair(vv + tb, p);
+        f64x2 r2 = {r[2 * p], r[2 * p + 1]};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = 2 * p + h;
+            if (q >= N3) break;  // padding dof of an odd 3N stays 0
             const int at = q / 3, d = q % 3;
             double acc = sld(qezm, at * 9 + 0 + d) * Ec[0] + sld(qezm, at * 9 + 3 + d) * Ec[1] +
                          sld(qezm, at * 9 + 6 + d) * Ec[2];
 #pragma unroll
             for (int jq = 0; jq < NMAX; ++jq)
                 if (jq < N3) acc -= sld(dmm, q * N3 + jq) * r[jq];
-            const double vq = ldnt(vv + tb + q * 64);
-            stnt(r[q] + (vq * dt + acc * hdt2), rr + tb + q * 64);
-            stnt(vq + acc * dt, vv + tb + q * 64);
+            const double vq = v2[h];
+            r2[h] = r[q] + (vq * dt + acc * hdt2);
+            v2[h] = vq + acc * dt;
         }
+        stpair(r2, rr + tb, p);
+        stpair(v2, vv + tb, p);
     }
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Precision form: lane l of a wavefront owns slot 64 tile + l and all 3N dofs
-// of its r^n and v^n - the memory pattern of the plain form (r[tile][dof][64],
-// every access one 512-byte row, all of r^n requested up front).  Only the
+// of its r^n and v^n - the memory pattern of the plain form (dof pairs as
+// 16-byte vectors, every access one 1 KiB row, all of r^n requested up front).  Only the
 // contraction -DMM . r^n runs on FP32 MFMA (v_mfma_f32_16x16x4f32; lane l on
 // gfx950: A[m][k] in lane m + 16k, B[k][n] in lane n + 16k, D[m][n] in lane
 // n + 16 (m / 4), element m % 4):
@@ -206,11 +230,17 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     }
     if (tile >= ntile) return;  // wave-uniform
     float *buf = lds_a + wave * WL;
-    const int64_t tb = tile * 64 * (int64_t)N3 + lane;  // dof q of this lane's slot at tb + 64 q
-    double r[N3X];
+    const int NP = (N3 + 1) / 2;
+    constexpr int NPX = (N3X + 1) / 2;
+    const int64_t tb = tile * 128 * (int64_t)NP + 2 * lane;  // dof pair p of this lane's slot at tb + 128 p
+    double r[2 * NPX];
 #pragma unroll
-    for (int q = 0; q < N3X; ++q)
-        if (q < N3) r[q] = ldnt(rr + tb + q * 64);
+    for (int p = 0; p < NPX; ++p)
+        if (p < NP) {
+            const f64x2 v2 = ldpair(rr + tb, p);
+            r[2 * p] = v2.x;
+            r[2 * p + 1] = v2.y;
+        }
     const int64_t slot = tile * 64 + lane;
     const bool live = slot < a.nslot;
     size_t cell = 0;
@@ -293,15 +323,24 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     const double dt = a.dt, hdt2 = a.half_dt2;
     const float *qm = qop + 9 * natom;
 #pragma unroll
-    for (int q = 0; q < N3X; ++q) {
-        if (q >= N3) break;
-        const int at = q / 3, d = q % 3;
-        const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] + sld(qm, at * 9 + 3 + d) * Ec[1] +
-                         sld(qm, at * 9 + 6 + d) * Ec[2];
-        const double am = (double)af;
-        const double vq = ldnt(vv + tb + q * 64);
-        stnt(r[q] + (vq * dt + am * hdt2), rr + tb + q * 64);
-        stnt(vq + am * dt, vv + tb + q * 64);
+    for (int p = 0; p < NPX; ++p) {
+        if (p >= NP) break;
+        f64x2 v2 = ldpair(vv + tb, p);
+        f64x2 r2 = {r[2 * p], r[2 * p + 1]};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = 2 * p + h;
+            if (q >= N3) break;  // padding dof of an odd 3N stays 0
+            const int at = q / 3, d = q % 3;
+            const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] +
+                             sld(qm, at * 9 + 3 + d) * Ec[1] + sld(qm, at * 9 + 6 + d) * Ec[2];
+            const double am = (double)af;
+            const double vq = v2[h];
+            r2[h] = r[q] + (vq * dt + am * hdt2);
+            v2[h] = vq + am * dt;
+        }
+        stpair(r2, rr + tb, p);
+        stpair(v2, vv + tb, p);
     }
 }
 

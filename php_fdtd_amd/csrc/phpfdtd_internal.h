@@ -92,21 +92,24 @@ inline int mfma_inst(int natom) {
 }
 
 // Lattice state layout of one material (r and v, FP64, both forms): tiles of
-// 64 slots, r[tile][dof][64] - a lane owns a slot and every wavefront access
-// is one 512-byte row of 64 slots of one dof.
+// 64 slots, dofs taken in pairs, r[tile][dof / 2][64][2] - a lane owns a slot
+// and reads the two dofs of a pair as one 16-byte vector, so every wavefront
+// access is one 1 KiB row.  An odd 3N is padded with one zero dof.
 struct IonLayout {
     int n3;
-    __host__ __device__ int64_t tile_r() const { return 64LL * n3; }
+    __host__ __device__ int n3p() const { return (n3 + 1) & ~1; }
+    __host__ __device__ int64_t tile_r() const { return 64LL * n3p(); }
     __host__ __device__ int64_t tile_v() const { return tile_r(); }
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
-        return (s >> 6) * tile_r() + (int64_t)q * 64 + (s & 63);
+        return (s >> 6) * tile_r() + (int64_t)(q >> 1) * 128 + (s & 63) * 2 + (q & 1);
     }
     __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
     // slot and dof of r element idx; false for padding
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
         const int64_t t = idx / tile_r(), rem = idx % tile_r();
-        *q = (int)(rem / 64);
-        *s = t * 64 + rem % 64;
+        const int w = (int)(rem % 128);
+        *q = (int)(rem / 128) * 2 + (w & 1);
+        *s = t * 64 + w / 2;
         return *q < n3 && *s < nslot;
     }
 };
