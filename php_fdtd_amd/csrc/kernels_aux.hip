@@ -5,25 +5,33 @@ namespace phpfdtd {
 
 // Per-step epilogue (one thread): the monitor row of fdtd.c:3806-3851 for the
 // step that just finished, then the device step counter advances.  The row is
-// written at index `step`, so graph replays need no host bookkeeping.
-template <typename real>
-__global__ void step_end(StepEndArgs a) {
-    if (threadIdx.x != 0) return;
+// written at index `step`, so graph replays need no host bookkeeping.  part 0
+// writes the whole row; part 1 only D and H (taken before the next step's H
+// update starts), part 2 t, E, R, V and advances the step.
+__device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool rest, const void *const *F, bool f64) {
     const int64_t st = *a.step;
     if (a.mon_on && st < a.ring_rows) {
         double *row = a.ring + st * 12;
-        row[0] = (double)st * a.dt;
-        for (int q = 0; q < 9; ++q)
-            row[1 + q] = a.mon_c >= 0 ? (double)static_cast<const real *>(a.F[q])[a.mon_c] : 0.0;
-        if (a.mon_ridx >= 0) {
-            row[10] = a.mon_r[a.mon_ridx];
-            row[11] = a.mon_v[a.mon_vidx];
-        } else {
-            row[10] = 0.0;
-            row[11] = 0.0;
+        auto val = [&](int q) -> double {
+            if (a.mon_c < 0) return 0.0;
+            return f64 ? static_cast<const double *>(F[q])[a.mon_c] : (double)static_cast<const float *>(F[q])[a.mon_c];
+        };
+        if (rest) {
+            row[0] = (double)st * a.dt;
+            for (int q = 0; q < 3; ++q) row[1 + q] = val(q);
+            row[10] = a.mon_ridx >= 0 ? a.mon_r[a.mon_ridx] : 0.0;
+            row[11] = a.mon_ridx >= 0 ? a.mon_v[a.mon_vidx] : 0.0;
         }
+        if (dh)
+            for (int q = 3; q < 9; ++q) row[1 + q] = val(q);
     }
-    *a.step = st + 1;
+    if (rest) *a.step = st + 1;
+}
+
+template <typename real>
+__global__ void step_end(StepEndArgs a) {
+    if (threadIdx.x != 0) return;
+    step_row(a, a.part != 2, a.part != 1, a.F, sizeof(real) == 8);
 }
 
 // step_end of every domain of a batch: work-group b handles domain b.
@@ -31,16 +39,7 @@ template <typename real>
 __global__ void step_end_batch(const StepEndArgs *args, int n) {
     if (threadIdx.x != 0 || (int)blockIdx.x >= n) return;
     const StepEndArgs a = args[blockIdx.x];
-    const int64_t st = *a.step;
-    if (a.mon_on && st < a.ring_rows) {
-        double *row = a.ring + st * 12;
-        row[0] = (double)st * a.dt;
-        for (int q = 0; q < 9; ++q)
-            row[1 + q] = a.mon_c >= 0 ? (double)static_cast<const real *>(a.F[q])[a.mon_c] : 0.0;
-        row[10] = a.mon_ridx >= 0 ? a.mon_r[a.mon_ridx] : 0.0;
-        row[11] = a.mon_ridx >= 0 ? a.mon_v[a.mon_vidx] : 0.0;
-    }
-    *a.step = st + 1;
+    step_row(a, true, true, a.F, sizeof(real) == 8);
 }
 
 template <typename T>

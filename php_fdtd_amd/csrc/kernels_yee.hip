@@ -88,8 +88,8 @@ __device__ __forceinline__ void h_tile(const YeeArgs<real> &a, const int t) {
     if (nr <= 0 || uniform(bx * 64 * VN) >= a.nx - 1) return;
     const bool lane_on = x0 < a.nx - 1;  // lanes past the row end still load and shuffle
     const int xr = x0 < a.nxp ? x0 : 0;  // in-row offset (lanes past the padded row read cell 0)
-    const int k0 = uniform(bz * a.kchunk);
-    const int k1 = min(k0 + a.kchunk, a.nz);
+    const int k0 = uniform(a.kbase + bz * a.kchunk);
+    const int k1 = min(k0 + a.kchunk, a.kend);
     const size_t nxp = a.nxp;
     const size_t plane = nxp * a.ny;
     const int L2 = 2 * a.L;
@@ -256,8 +256,8 @@ __device__ __forceinline__ void de_tile(const YeeArgs<real> &a, const int t, con
     if (nr <= 0 || uniform(bx * 64 * VN) >= a.nx) return;
     const bool lane_on = x0 < a.nx;
     const int xr = x0 < a.nxp ? x0 : 0;
-    const int k0 = uniform(bz * a.kchunk);
-    const int k1 = min(k0 + a.kchunk, a.nz);
+    const int k0 = uniform(a.kbase + bz * a.kchunk);
+    const int k1 = min(k0 + a.kchunk, a.kend);
     const size_t nxp = a.nxp;
     const size_t plane = nxp * a.ny;
     const int L2 = 2 * a.L;

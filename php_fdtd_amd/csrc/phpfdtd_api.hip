@@ -84,9 +84,15 @@ struct Special {
     SlotArgs<float> sa32{};
 };
 
+// Stages of one time step, in issue order.  With overlap, the H ranges that
+// read no lattice-cell E (K_HA) of step n+1 run on a second stream beside the
+// lattice kernels (K_ION) of step n; every other order is sequential.
+enum Stage { K_HA, K_HB, K_DE, K_PRE, K_ION, K_POST };
+
 struct Kernel {
     std::string name;
     double bytes = 0, flops = 0;
+    int stage = K_DE;
     std::function<void(hipStream_t)> fn;
 };
 
@@ -140,6 +146,7 @@ struct phpfdtd_ctx {
     float *d_slice = nullptr;
     // near-field DFT of one plane
     bool dft_on = false;
+    int dft_comp = 0;
     DftArgs dft{};
     double2 *d_dft_w = nullptr, *d_dft_acc = nullptr;
     // argument blocks of the current plan (batched launches read these)
@@ -151,6 +158,10 @@ struct phpfdtd_ctx {
     hipEvent_t ev = nullptr;  // marks this stream's latest work for the batch stream
     // plan and graphs
     std::vector<Kernel> plan;
+    bool allow_overlap = true;            // PHPFDTD_NO_OVERLAP=1 turns the overlap off
+    bool overlap = false;                 // plan has K_HA ranges to overlap with K_ION
+    hipStream_t stream2 = nullptr;        // second stream of the overlapped graphs
+    std::vector<hipEvent_t> cap_events;   // fork/join events of the captured graphs
     hipGraphExec_t g1 = nullptr, g16 = nullptr;
     bool dirty = true;
     bool use_graph = true;
@@ -161,9 +172,11 @@ struct phpfdtd_ctx {
         if (stream) (void)hipStreamSynchronize(stream);
         if (g1) hipGraphExecDestroy(g1);
         if (g16) hipGraphExecDestroy(g16);
-        if (ev) hipEventDestroy(ev);
-        for (void *p : allocs) hipFree(p);
-        if (stream) hipStreamDestroy(stream);
+        if (ev) (void)hipEventDestroy(ev);
+        for (hipEvent_t e : cap_events) (void)hipEventDestroy(e);
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream2) (void)hipStreamDestroy(stream2);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 
     int alloc(void **p, size_t n) {
@@ -327,6 +340,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
     const int nbz = (x->nz + kc - 1) / kc;
     const int bxy = a.nbx * a.nby;
     a.kchunk = kc;
+    a.kbase = 0;
+    a.kend = x->nz;
     a.nbz = nbz;
     a.nblocks = bxy * nbz;
     a.nblocks_pad = (a.nblocks + 7) / 8 * 8;
@@ -347,12 +362,42 @@ static void build_plan_t(phpfdtd_ctx *x) {
     double special_cells = 0;
     for (const auto &sp : x->specials) special_cells += (double)sp.cells.size();
     const double lossy_int = (double)x->lossy_interior;
-    Kernel kh;
-    kh.name = "h_update";
-    kh.bytes = hcells * 9 * w + pml_inc(nx - 1, ny - 1, nz) * 4 * w;
-    kh.flops = hcells * 18;
-    kh.fn = [a](hipStream_t s) { launch_h<real>(a, s); };
-    x->plan.push_back(kh);
+    // H over z ranges: with a lattice material, planes [klo, khi) read E of
+    // lattice cells (plane k reads E at k and k+1); the others do not and may
+    // run beside the previous step's lattice kernels
+    int klo = x->nz, khi = 0;
+    for (const auto &sp : x->specials)
+        if (sp.pre.natom > 0 && !sp.cells.empty()) {
+            klo = std::min(klo, (int)(sp.cells.front() / ((size_t)x->nxp * x->ny)) - 1);
+            khi = std::max(khi, (int)(sp.cells.back() / ((size_t)x->nxp * x->ny)) + 1);
+        }
+    klo = std::max(klo, 0);
+    khi = std::min(khi, x->nz);
+    x->overlap = x->allow_overlap && khi > klo && (klo > 0 || khi < x->nz);
+    auto h_range = [&](int kb, int ke, int stage, const char *nm) {
+        if (ke <= kb) return;
+        YeeArgs<real> r = a;
+        r.kbase = kb;
+        r.kend = ke;
+        r.nbz = (ke - kb + kc - 1) / kc;
+        r.nblocks = bxy * r.nbz;
+        r.nblocks_pad = (r.nblocks + 7) / 8 * 8;
+        const double f = (double)(ke - kb) / nz;
+        Kernel kh;
+        kh.name = nm;
+        kh.stage = stage;
+        kh.bytes = f * (hcells * 9 * w + pml_inc(nx - 1, ny - 1, nz) * 4 * w);
+        kh.flops = f * hcells * 18;
+        kh.fn = [r](hipStream_t s) { launch_h<real>(r, s); };
+        x->plan.push_back(kh);
+    };
+    if (x->overlap) {
+        h_range(0, klo, K_HA, "h_update[lo]");
+        h_range(khi, x->nz, K_HA, "h_update[hi]");
+        h_range(klo, khi, K_HB, "h_update[lattice]");
+    } else {
+        h_range(0, x->nz, K_HB, "h_update");
+    }
     Kernel kd;
     kd.name = "de_update";
     kd.bytes = cells * (6 * w + 1) + (cells - special_cells) * 3 * w + icells * 3 * w + lossy_int * 3 * w +
@@ -393,6 +438,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
             snprintf(nm, sizeof(nm), "ion_update[m%d,N%d]", sp.id, sp.pre.natom);
             k.bytes = n * (6 * w + 4 + 4 * N3 * 8 + (sp.pre.ade ? 12 * w : 0));
             k.flops = n * (2 * N3 * N3 + 12 * N3 + 40);
+            k.stage = K_ION;
             k.fn = [sa](hipStream_t s) { launch_ion<real>(sa, s); };
         } else {
             snprintf(nm, sizeof(nm), "ade_update[m%d]", sp.id);
@@ -410,6 +456,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
         kf.name = "dft_accum";
         kf.bytes = (double)da.nplane * (w + 32);
         kf.flops = (double)da.nplane * 4;
+        // D and H are final before the next H update, E only after the lattice kernels
+        kf.stage = (x->dft_comp >= PHPFDTD_DX && x->dft_comp <= PHPFDTD_HZ) ? K_PRE : K_POST;
         kf.fn = [da](hipStream_t s) { launch_dft<real>(da, s); };
         x->plan.push_back(kf);
     }
@@ -425,11 +473,31 @@ static void build_plan_t(phpfdtd_ctx *x) {
     se.mon_ridx = x->mon_ridx;
     se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
+    se.part = 0;
     x->se = se;
-    Kernel ke;
-    ke.name = "step_end";
-    ke.fn = [se](hipStream_t s) { launch_step_end<real>(se, s); };
-    x->plan.push_back(ke);
+    if (x->overlap) {
+        StepEndArgs s1 = se, s2 = se;
+        s1.part = 1;
+        s2.part = 2;
+        Kernel k1, k2;
+        k1.name = "step_end[dh]";
+        k1.stage = K_PRE;
+        k1.fn = [s1](hipStream_t s) { launch_step_end<real>(s1, s); };
+        k2.name = "step_end";
+        k2.stage = K_POST;
+        k2.fn = [s2](hipStream_t s) { launch_step_end<real>(s2, s); };
+        x->plan.push_back(k1);
+        x->plan.push_back(k2);
+    } else {
+        Kernel ke;
+        ke.name = "step_end";
+        ke.stage = K_POST;
+        ke.fn = [se](hipStream_t s) { launch_step_end<real>(se, s); };
+        x->plan.push_back(ke);
+    }
+    // issue order of one step
+    std::stable_sort(x->plan.begin(), x->plan.end(),
+                     [](const Kernel &p, const Kernel &q) { return p.stage < q.stage; });
 }
 
 static void build_plan(phpfdtd_ctx *x) {
@@ -449,8 +517,38 @@ static int rebuild(phpfdtd_ctx *x) {
         const int n = which == 0 ? 1 : 16;
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
-        for (int t = 0; t < n; ++t)
-            for (auto &k : x->plan) k.fn(x->stream);
+        if (!x->overlap) {
+            for (int t = 0; t < n; ++t)
+                for (auto &k : x->plan) k.fn(x->stream);
+        } else {
+            // step t+1's free H ranges fork onto stream2 after step t's
+            // D/H sampling and join before step t's epilogue part 2
+            while ((int)x->cap_events.size() < 2 * n) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                x->cap_events.push_back(e);
+            }
+            for (auto &k : x->plan)
+                if (k.stage == K_HA) k.fn(x->stream);
+            for (int t = 0; t < n; ++t) {
+                const bool fork = t + 1 < n;
+                for (auto &k : x->plan) {
+                    if (k.stage == K_HA) continue;
+                    if (k.stage == K_ION && fork && &k == &*std::find_if(x->plan.begin(), x->plan.end(),
+                                                                         [](const Kernel &q) { return q.stage == K_ION; })) {
+                        HIPCHK(hipEventRecord(x->cap_events[2 * t], x->stream));
+                        HIPCHK(hipStreamWaitEvent(x->stream2, x->cap_events[2 * t], 0));
+                        for (auto &h : x->plan)
+                            if (h.stage == K_HA) h.fn(x->stream2);
+                        HIPCHK(hipEventRecord(x->cap_events[2 * t + 1], x->stream2));
+                    }
+                    if (k.stage == K_POST && fork && &k == &*std::find_if(x->plan.begin(), x->plan.end(),
+                                                                          [](const Kernel &q) { return q.stage == K_POST; }))
+                        HIPCHK(hipStreamWaitEvent(x->stream, x->cap_events[2 * t + 1], 0));
+                    k.fn(x->stream);
+                }
+            }
+        }
         HIPCHK(hipStreamEndCapture(x->stream, &graph));
         hipGraphExec_t ex = nullptr;
         hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
@@ -520,6 +618,8 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     x->nmat = nmat;
     const char *ng = getenv("PHPFDTD_NO_GRAPH");
     x->use_graph = !(ng && ng[0] == '1');
+    const char *no = getenv("PHPFDTD_NO_OVERLAP");
+    x->allow_overlap = !(no && no[0] == '1');
     // Yee tile shape (defaults measured on MI355X, 1024x1024x640); overridable for tuning
     x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 8 : 16;
     if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
@@ -538,6 +638,7 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     }
     HIPCHK(hipGetDevice(&x->device));
     HIPCHK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&x->stream2, hipStreamNonBlocking));
 
     std::vector<int> natom(nmat), ade(nmat);
     for (int m = 0; m < nmat; ++m) {
@@ -811,6 +912,7 @@ int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat
     a.w = x->d_dft_w;
     a.acc = x->d_dft_acc;
     a.step = x->d_step;
+    x->dft_comp = comp;
     x->dft_on = true;
     touch(x);
     if (bin) *bin = k;

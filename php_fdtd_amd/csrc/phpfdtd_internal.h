@@ -68,6 +68,7 @@ struct YeeArgs {
     int nx, ny, nz, nxp, L;
     int nmat;  // entries of mats
     int rows;  // y rows per wavefront (1, 2 or 4)
+    int kbase, kend;  // z planes [kbase, kend) of this launch
     int kchunk, nbx, nby, nbz, nblocks, nblocks_pad;
     // sources: cell and component (i, j, k, 0|1|2) in src[4s..4s+3], value*dt
     // per step in src_val[step*nsrc + s]
@@ -146,7 +147,10 @@ struct SlotArgs {
     double dt, half_dt2;
 };
 
-// Per-step epilogue: monitor row + step counter.
+// Per-step epilogue: monitor row + step counter.  When the next step's H
+// update overlaps the lattice kernel, the row is written in two parts: D and
+// H (part 1) before the overlap, t, E, R, V and the step advance (part 2)
+// after it.
 struct StepEndArgs {
     int64_t *step;
     double *ring;      // [tsteps][12]
@@ -157,6 +161,7 @@ struct StepEndArgs {
     const double *mon_r, *mon_v;  // lattice state of the monitor cell's material
     int64_t mon_ridx, mon_vidx;   // offsets of (slot, atom*3+dir) in r and v, -1 = no R/V
     double dt;
+    int part;                     // 0 whole row, 1 D and H, 2 t, E, R, V + step advance
 };
 
 // One launch over the work of several domains (a batch): tile g of the grid
