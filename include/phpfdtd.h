@@ -152,6 +152,10 @@ int phpfdtd_create(const phpfdtd_grid *grid, const phpfdtd_material *mats, int n
  * are evaluated on the host once. */
 int phpfdtd_set_sources(phpfdtd_ctx *ctx, int n, const phpfdtd_source *src);
 
+/* calculate_source_value (fdtd.c:1923-1947) at time t in FP64, the value the
+ * library injects (times dt); for drivers writing source.txt (fdtd.c:3359-3367). */
+double phpfdtd_source_value(const phpfdtd_source *src, double t);
+
 /* Enable the per-step monitor ring (fdtd.c:3806-3883).  ref_compat=1 samples
  * E, D and H at plane z+1 as the reference does (its field pointers are
  * pre-offset by the halo, fdtd.c:274 with 3814) and r, v at plane z.  Each row

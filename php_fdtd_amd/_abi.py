@@ -199,6 +199,8 @@ def load_library(path=None):
                                             P(ctypes.c_size_t)]
     lib.phpfdtd_profile.argtypes = [c_ctx, ctypes.c_int, P(KTime), ctypes.c_int, P(ctypes.c_int)]
     lib.phpfdtd_time_steps.argtypes = [c_ctx, ctypes.c_int, P(ctypes.c_double)]
+    lib.phpfdtd_source_value.argtypes = [P(Source), ctypes.c_double]
+    lib.phpfdtd_source_value.restype = ctypes.c_double
     lib.phpfdtd_step.argtypes = [c_ctx]
     lib.phpfdtd_step.restype = ctypes.c_int64
     lib.phpfdtd_last_error.restype = ctypes.c_char_p
@@ -218,7 +220,7 @@ EXPORTED = [
     "phpfdtd_batch_run", "phpfdtd_batch_sync", "phpfdtd_batch_wait", "phpfdtd_batch_size", "phpfdtd_batch_destroy",
     "phpfdtd_read_field", "phpfdtd_write_field", "phpfdtd_count_cells", "phpfdtd_read_state",
     "phpfdtd_write_state", "phpfdtd_footprint_bytes", "phpfdtd_profile", "phpfdtd_time_steps",
-    "phpfdtd_step", "phpfdtd_last_error", "phpfdtd_abi_version", "phpfdtd_destroy",
+    "phpfdtd_step", "phpfdtd_last_error", "phpfdtd_abi_version", "phpfdtd_destroy", "phpfdtd_source_value",
 ]
 
 

@@ -582,6 +582,8 @@ extern "C" {
 
 int phpfdtd_abi_version(void) { return PHPFDTD_ABI_VERSION; }
 
+double phpfdtd_source_value(const phpfdtd_source *src, double t) { return src ? source_value(*src, t) : 0.0; }
+
 const char *phpfdtd_last_error(void) { return g_err.c_str(); }
 
 int phpfdtd_footprint_bytes(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
