@@ -19,11 +19,18 @@ Configurations (sizes as named):
   cfg4  serving mix: grids {64^3, 96^3, 128^3, 192x192x128, 256x256x160},
         materials {cfg0-, cfg1-, cfg2-type}, source U(600, 1600) cm^-1,
         steps U(5000, 20000), 10-cell CPML, Poisson arrivals.
+
+Every medium is drawn by ``php_fdtd_amd.medium`` from a ``[medium]`` geometry
+(cubes, plus a cone for cfg3) kept in ``spec["medium"]``; the cfg written by
+``write_reference_case`` carries it, so the same map can be redrawn from the
+cfg alone.
 """
 import math
 import os
 
 import numpy as np
+
+from . import medium as _medium
 
 C0 = 3e8
 EPS0 = 8.854e-12
@@ -136,6 +143,18 @@ def _source(kind, pos, comp, f, t0=None, tau=None, amplitude=1.0, tsteps=0, dt=0
                 start_time=0.0, end_time=tsteps * dt)
 
 
+def _slab(n, z0, z1, material=4):
+    """Cube covering the whole x-y extent over z0 <= z < z1."""
+    return dict(xrange=(0, n[0] - 1), yrange=(0, n[1] - 1), zrange=(z0, z1 - 1), material=material)
+
+
+def _draw(spec, geometry):
+    """Attach the [medium] geometry to spec and draw it on the spec's grid."""
+    g = spec["grid"]
+    spec["medium"] = geometry
+    return _medium.build((g["nx"], g["ny"], g["nz"]), (g["dx"], g["dy"], g["dz"]), geometry)
+
+
 def material_cfg0(f):
     er, ei = lorentz_eps(f / WN_TO_HZ, 2.5, 1370.0, 2.0, 5.0)
     m = vacuum()
@@ -172,49 +191,38 @@ def material_cfg2(seed=2):
 def config0():
     f = 650.0 * WN_TO_HZ
     g = _grid((100, 100, 100), 20e-9, 2000, 10, f)
-    med = np.zeros((100, 100, 100), np.int32)
-    med[:, :, 40:60] = 4
     mats = builtin_table(f) + [material_cfg0(f)]
     src = [_source("ricker", (50, 50, 64), "z", f, t0=5.128e-14, tsteps=2000, dt=g["dt"])]
-    return dict(name="cfg0", grid=g, materials=mats, sources=src, monitor=dict(x=50, y=50, z=69, atom=0, dir=2)), med
+    spec = dict(name="cfg0", grid=g, materials=mats, sources=src, monitor=dict(x=50, y=50, z=69, atom=0, dir=2))
+    return spec, _draw(spec, dict(cubes=[_slab((100, 100, 100), 40, 60)]))
 
 
 def config1():
     f = 1400.0 * WN_TO_HZ
     g = _grid((160, 160, 120), 20e-9, 20000, 12, f)
-    med = np.zeros((160, 160, 120), np.int32)
-    med[:, :, 50:70] = 4
     mats = builtin_table(f) + [material_cfg1()]
     src = [_source("dipole", (80, 80, 45), "z", f, tsteps=20000, dt=g["dt"])]
-    return dict(name="cfg1", grid=g, materials=mats, sources=src, monitor=dict(x=80, y=80, z=60, atom=0, dir=0)), med
+    spec = dict(name="cfg1", grid=g, materials=mats, sources=src, monitor=dict(x=80, y=80, z=60, atom=0, dir=0))
+    return spec, _draw(spec, dict(cubes=[_slab((160, 160, 120), 50, 70)]))
 
 
 def config2():
     f = 800.0 * WN_TO_HZ
     g = _grid((384, 384, 256), 20e-9, 30000, 16, f)
-    med = np.zeros((384, 384, 256), np.int32)
-    med[:, :, 112:144] = 4
     mats = builtin_table(f) + [material_cfg2()]
     src = [_source("dipole", (192, 192, 100), "z", f, tsteps=30000, dt=g["dt"])]
-    return dict(name="cfg2", grid=g, materials=mats, sources=src,
-                monitor=dict(x=192, y=192, z=128, atom=0, dir=0)), med
+    spec = dict(name="cfg2", grid=g, materials=mats, sources=src, monitor=dict(x=192, y=192, z=128, atom=0, dir=0))
+    return spec, _draw(spec, dict(cubes=[_slab((384, 384, 256), 112, 144)]))
 
 
-def cone_medium(n, slab, apex_z, apex_r, half_angle_deg, top_z, slab_id=4, cone_id=5, center=None):
-    """Medium with a z-slab [slab0, slab1) and a cone opening upward from apex
-    height apex_z (radius apex_r cells) to top_z (exclusive)."""
-    nx, ny, nz = n
-    med = np.zeros((nx, ny, nz), np.int32)
-    med[:, :, slab[0]:slab[1]] = slab_id
-    cx, cy = center if center else (nx // 2, ny // 2)
-    x = np.arange(nx)[:, None] - cx
-    y = np.arange(ny)[None, :] - cy
-    r2 = x * x + y * y
+def tip_cone(n, apex_z, apex_r, half_angle_deg, top_z, d, material=5, center=None):
+    """Cone entry of the [medium] geometry: axis along +z from (cx, cy, apex_z)
+    to (cx, cy, top_z - 1), radius apex_r cells at the apex growing by
+    tan(half angle) per cell, in metres for cell size d."""
+    cx, cy = center if center else (n[0] // 2, n[1] // 2)
     t = math.tan(math.radians(half_angle_deg))
-    for z in range(apex_z, top_z):
-        rad = apex_r + (z - apex_z) * t
-        med[:, :, z][r2 <= rad * rad] = cone_id
-    return med
+    return dict(p1=(cx, cy, apex_z), p2=(cx, cy, top_z - 1), r1=apex_r * d, r2=(apex_r + (top_z - 1 - apex_z) * t) * d,
+                material=material)
 
 
 def material_drude_tip(f):
@@ -234,12 +242,14 @@ def config3(scale=1):
     slab0 = n[2] * 200 // 640
     slab1 = slab0 + max(2, 48 // scale)
     apex = slab1 + max(1, 3 // scale)
-    med = cone_medium(n, (slab0, slab1), apex, max(1, 10 // scale), 30.0, n[2] - L)
     mats = builtin_table(f) + [material_cfg2(), material_drude_tip(f)]
     src_z = slab1 + max(0, (apex - slab1) // 2)
     src = [_source("ricker", (n[0] // 2, n[1] // 2, src_z), "z", f, t0=5e-14, tsteps=10000, dt=g["dt"])]
-    return dict(name="cfg3" if scale == 1 else "cfg3/%d" % scale, grid=g, materials=mats, sources=src,
-                monitor=dict(x=n[0] // 2, y=n[1] // 2, z=slab0 + (slab1 - slab0) // 2, atom=0, dir=2)), med
+    spec = dict(name="cfg3" if scale == 1 else "cfg3/%d" % scale, grid=g, materials=mats, sources=src,
+                monitor=dict(x=n[0] // 2, y=n[1] // 2, z=slab0 + (slab1 - slab0) // 2, atom=0, dir=2))
+    geometry = dict(cubes=[_slab(n, slab0, slab1)],
+                    cones=[tip_cone(n, apex, max(1, 10 // scale), 30.0, n[2] - L, g["dx"])])
+    return spec, _draw(spec, geometry)
 
 
 CONFIGS = {"cfg0": config0, "cfg1": config1, "cfg2": config2, "cfg3": config3}
@@ -278,8 +288,6 @@ def serving_case(req):
     g = _grid((nx, ny, nz), 20e-9, req["steps"], 10, f)
     s0 = nz * 5 // 12
     s1 = nz * 7 // 12
-    med = np.zeros((nx, ny, nz), np.int32)
-    med[:, :, s0:s1] = 4
     kind = req["kind"]
     if kind == "lorentz":
         mat = material_cfg0(f)
@@ -295,7 +303,7 @@ def serving_case(req):
     src = [_source("dipole", (nx // 2, ny // 2, s0 - 5), "z", f, tsteps=req["steps"], dt=g["dt"])]
     spec = dict(name="req%d/%s/%dx%dx%d" % (req["id"], kind, nx, ny, nz), grid=g, materials=builtin_table(f) + [mat],
                 sources=src, monitor=dict(x=nx // 2, y=ny // 2, z=s1 + 3, atom=0, dir=2))
-    return spec, med
+    return spec, _draw(spec, dict(cubes=[_slab((nx, ny, nz), s0, s1)]))
 
 
 def with_steps(spec, steps):
@@ -337,10 +345,7 @@ def step_bytes(spec, med, word):
 # reference-format writers (for running the same workload through fdtd.c)
 # --------------------------------------------------------------------------
 def write_medium(path, med):
-    nx, ny, nz = med.shape
-    with open(path, "wb") as f:
-        f.write(np.array([nx, ny, nz], dtype="<i4").tobytes())
-        f.write(np.ascontiguousarray(med, dtype="<i4").tobytes())
+    _medium.save_medium_file(med, path)
 
 
 def _fmt(v):
@@ -426,5 +431,9 @@ def write_reference_case(d, spec, med, output_interval=None, with_born=True):
         lines.append("monitor_point = x=%d,y=%d,z=%d,atom=%d,dir=%s" % (mon["x"], mon["y"], mon["z"], mon["atom"],
                                                                        "xyz"[mon["dir"]]))
         lines.append("monitor_output_file = monitor_point.dat")
-    open(os.path.join(d, "case.cfg"), "w").write("\n".join(lines) + "\n")
+    text = "\n".join(lines) + "\n"
+    if spec.get("medium"):
+        # the geometry that draws medium_map.bin: python3 -m php_fdtd_amd.medium case.cfg
+        text += "\n" + _medium.format_section(spec["medium"])
+    open(os.path.join(d, "case.cfg"), "w").write(text)
     return os.path.join(d, "case.cfg")
