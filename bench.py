@@ -83,7 +83,8 @@ def accuracy(name, interval=10):
     f = spec["sources"][-1]["frequency"]
     mon = spec["monitor"]
     res = {}
-    for prec in (_abi.FP32, _abi.FP64):
+    names = {_abi.FP32: "fp32", _abi.FP32_RV64: "fp32 (r,v fp64)", _abi.FP64: "fp64"}
+    for prec in (_abi.FP32, _abi.FP32_RV64, _abi.FP64):
         sim = Simulation(spec, med, prec)
         sim.set_monitor(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"], ref_compat=True)
         k = sim.set_dft("Ez", "z", zplane, interval, N, f, ref_compat=False)
@@ -96,8 +97,8 @@ def accuracy(name, interval=10):
         d = sim.sync()
         sim.close()
         res[prec] = dict(rows=rows, x=x, k=k, wall=wall, max_r=d.max_r)
-        log("  accuracy %s %s: %d steps in %.2f s" % (name, "fp32" if prec else "fp64", T, wall))
-    a, b = res[_abi.FP32], res[_abi.FP64]
+        log("  accuracy %s %s: %d steps in %.2f s" % (name, names[prec], T, wall))
+    a, b, c = res[_abi.FP32], res[_abi.FP64], res[_abi.FP32_RV64]
     x32, x64 = central(a["x"]), central(b["x"])
     amp32, amp64 = np.abs(x32), np.abs(x64)
     ph = np.angle(x32 * np.conj(x64))
@@ -113,12 +114,15 @@ def accuracy(name, interval=10):
             name, g["nx"], g["ny"], g["nz"], T),
         "monitor_rel_l2": rl2(a["rows"][:, 1:10], b["rows"][:, 1:10]),
         "monitor_rv_rel_l2": rl2(a["rows"][:, 10:12], b["rows"][:, 10:12]),
+        "rv_fp64_storage": {"monitor_rel_l2": rl2(c["rows"][:, 1:10], b["rows"][:, 1:10]),
+                            "monitor_rv_rel_l2": rl2(c["rows"][:, 10:12], b["rows"][:, 10:12]),
+                            "vs_compensated_monitor_rel_l2": rl2(a["rows"][:, 1:12], c["rows"][:, 1:12])},
         "nearfield": {"component": "Ez", "plane": "z=%d" % zplane, "samples": N, "interval": interval,
                       "bin": b["k"], "bin_hz": b["k"] * fs / N, "source_hz": f,
                       "amp_rel_l2": rl2(amp32, amp64),
                       "phase_rms_rad": float(np.sqrt(np.sum(amp64 ** 2 * ph ** 2) / np.sum(amp64 ** 2))),
                       "phase_max_rad_strong": float(np.abs(ph[strong]).max())},
-        "solve_s": {"fp32": round(a["wall"], 3), "fp64": round(b["wall"], 3)},
+        "solve_s": {"fp32": round(a["wall"], 3), "fp32_rv64": round(c["wall"], 3), "fp64": round(b["wall"], 3)},
     }
 
 
@@ -238,7 +242,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="cfg3")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,fp64")
+    ap.add_argument("--forms", default="fp32,fp64", help="comma list of fp32, fp32rv64, fp64")
     ap.add_argument("--accuracy", default="cfg1", help="config for the FP32-vs-FP64 accuracy run ('' = skip)")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving mix (0 = skip)")
     ap.add_argument("--serving-step-scale", type=float, default=0.05)
@@ -257,21 +261,24 @@ def main():
     log("workload %s: %s cells, built in %.1f s" % (spec["name"], med.size, time.time() - t0))
     cells = float(med.size)
     res = {}
+    precs = {"fp32": _abi.FP32, "fp32rv64": _abi.FP32_RV64, "fp64": _abi.FP64}
+    rv_bytes = {"fp32": 5, "fp32rv64": 8, "fp64": 8}
     for form in args.forms.split(","):
-        prec = _abi.FP32 if form == "fp32" else _abi.FP64
+        prec = precs[form]
         log("form %s" % form)
         res[form] = run_form(spec, med, prec, args.warmup, args.steps, prof_steps)
         log("  %.3f ms/step (events), %.3f ms/step (wall)" % (res[form]["ms"], res[form]["wall_ms"]))
         for k in res[form]["prof"]:
             log("    %-24s %9.3f ms  %8.1f GB/s" % (k["name"], k["ms"], k["bytes"] / k["ms"] / 1e6 if k["ms"] else 0))
 
-    head = "fp32" if "fp32" in res else "fp64"
+    head = [f for f in ("fp32", "fp32rv64", "fp64") if f in res][0]
     r = res[head]
-    word = 4 if head == "fp32" else 8
+    word = 8 if head == "fp64" else 4
     gcell = cells / (r["ms"] * 1e-3) / 1e9
     dom = max(r["prof"], key=lambda k: k["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
-    sb = workloads.step_bytes(spec, med, word)
+    sb = workloads.step_bytes(spec, med, word, rv_bytes[head])
+    rv_desc = {"fp32": "compensated f32 (f64 integration)", "fp32rv64": "f64", "fp64": "f64"}[head]
     out = {
         "metric": METRIC,
         "value": round(gcell, 4),
@@ -283,17 +290,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if head == "fp32" else "f64",
+        "dtype": "f64" if head == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": "%s: %dx%dx%d cells, 16-atom lattice slab + Drude cone, %s form (fields/CPML/ADE "
-                               "%s, lattice r,v f64)" % (spec["name"], spec["grid"]["nx"], spec["grid"]["ny"],
-                                                        spec["grid"]["nz"], head.upper(), head),
+                               "%s, lattice r,v %s)" % (spec["name"], spec["grid"]["nx"], spec["grid"]["ny"],
+                                                       spec["grid"]["nz"], head.upper(), "f64" if head == "fp64" else
+                                                       "f32", rv_desc),
                    "cells": int(cells), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "step_roofline": {"bytes_per_step": sb, "achieved": round(sb / (r["ms"] * 1e-3) / 1e9, 1),
                           "frac": round(sb / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "model": "C(18w+1) + 8w*CPML incidences + 9w*ADE cells + 96N*lattice cells"},
+                          "model": "C(18w+1) + 8w*CPML incidences + 9w*ADE cells + 12Nb*lattice cells, b = %d "
+                                   "bytes per stored r or v value" % rv_bytes[head]},
         "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r["prof"]},
     }
     if "fp64" in res and head != "fp64":
@@ -302,7 +311,11 @@ def main():
                        "step_frac": round(workloads.step_bytes(spec, med, 8) / (r64["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r64["prof"]}}
         out["speedup_fp32_vs_fp64"] = round(r64["ms"] / r["ms"], 3)
-        a, b = res["fp32"]["rows"], r64["rows"]
+        if "fp32rv64" in res and head != "fp32rv64":
+            rr = res["fp32rv64"]
+            out["fp32_rv64"] = {"value": round(cells / (rr["ms"] * 1e-3) / 1e9, 4), "ms_per_step": round(rr["ms"], 4),
+                                "kernels_ms": {k["name"]: round(k["ms"], 4) for k in rr["prof"]}}
+        a, b = r["rows"], r64["rows"]
         n = min(len(a), len(b))
         if n:
             num = np.linalg.norm(a[:n, 1:10] - b[:n, 1:10])

@@ -57,8 +57,11 @@ extern "C" {
 
 /* Numerical forms. */
 #define PHPFDTD_FP64 0  /* plain form: everything in FP64                          */
-#define PHPFDTD_FP32 1  /* precision form: fields, CPML, ADE in FP32; ion r,v FP64,
-                           DMM.r on FP32 MFMA with FP32 accumulation              */
+#define PHPFDTD_FP32 1  /* precision form: fields, CPML, ADE in FP32; DMM.r on FP32
+                           MFMA with FP32 accumulation; ion r,v integrated in FP64
+                           and stored as compensated FP32 (an FP32 word plus an
+                           8-bit compensation term: 32 significant bits)         */
+#define PHPFDTD_FP32_RV64 2  /* precision form with ion r,v stored in FP64      */
 
 /* Field components (phpfdtd_read_field / write_field / read_slice). */
 #define PHPFDTD_EX 0
@@ -81,7 +84,7 @@ typedef struct phpfdtd_grid {
     double  dt;               /* s, already CFL-clamped (fdtd.c:1906-1910)          */
     double  eps0, mu0;        /* 8.854e-12, 4*pi*1e-7 in the reference (1751-1752)  */
     int32_t pml_thickness;    /* L                                                  */
-    int32_t precision;        /* PHPFDTD_FP64 or PHPFDTD_FP32                       */
+    int32_t precision;        /* PHPFDTD_FP64, PHPFDTD_FP32 or PHPFDTD_FP32_RV64    */
     double  pml_reflection;   /* R                                                  */
     double  pml_order;        /* m, 3 in the reference                              */
     double  kappa_max;        /* 7 in the reference                                 */

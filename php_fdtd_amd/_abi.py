@@ -22,7 +22,8 @@ EFORMAT = -5
 ESTATE = -6
 
 FP64 = 0
-FP32 = 1
+FP32 = 1       # precision form, lattice r, v stored as compensated FP32
+FP32_RV64 = 2  # precision form, lattice r, v stored in FP64
 
 COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Dx": 3, "Dy": 4, "Dz": 5, "Hx": 6, "Hy": 7, "Hz": 8, "E": 9}
 

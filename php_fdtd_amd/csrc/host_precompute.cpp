@@ -255,8 +255,8 @@ int validate_grid(const phpfdtd_grid &g, std::string *err) {
         *err = "pml_reflection must be in (0, 1)";
         return PHPFDTD_EINVAL;
     }
-    if (g.precision != PHPFDTD_FP64 && g.precision != PHPFDTD_FP32) {
-        *err = "precision must be PHPFDTD_FP64 or PHPFDTD_FP32";
+    if (g.precision != PHPFDTD_FP64 && g.precision != PHPFDTD_FP32 && g.precision != PHPFDTD_FP32_RV64) {
+        *err = "precision must be PHPFDTD_FP64, PHPFDTD_FP32 or PHPFDTD_FP32_RV64";
         return PHPFDTD_EINVAL;
     }
     return 0;

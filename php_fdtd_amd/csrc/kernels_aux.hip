@@ -19,8 +19,18 @@ __device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool res
         if (rest) {
             row[0] = (double)st * a.dt;
             for (int q = 0; q < 3; ++q) row[1 + q] = val(q);
-            row[10] = a.mon_ridx >= 0 ? a.mon_r[a.mon_ridx] : 0.0;
-            row[11] = a.mon_ridx >= 0 ? a.mon_v[a.mon_vidx] : 0.0;
+            double rv[2] = {0.0, 0.0};
+            if (a.mon_ridx >= 0) {
+                if (a.mon_rl) {  // compensated FP32 state
+                    rv[0] = pk_unpack(static_cast<const float *>(a.mon_r)[a.mon_ridx], a.mon_rl[a.mon_ridx]);
+                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx], a.mon_vl[a.mon_vidx]);
+                } else {
+                    rv[0] = static_cast<const double *>(a.mon_r)[a.mon_ridx];
+                    rv[1] = static_cast<const double *>(a.mon_v)[a.mon_vidx];
+                }
+            }
+            row[10] = rv[0];
+            row[11] = rv[1];
         }
         if (dh)
             for (int q = 3; q < 9; ++q) row[1 + q] = val(q);
@@ -80,11 +90,13 @@ __global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, 
     }
 }
 
-// max |r| over one lattice material's state.
-__global__ __launch_bounds__(256) void reduce_r(const double *r, int64_t n, unsigned long long *out) {
+// max |r| over one lattice material's state (FP64 values, or the FP32 words
+// of compensated state: max |r| rounded to FP32).
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_r(const T *r, int64_t n, unsigned long long *out) {
     unsigned long long m = 0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long b = __double_as_longlong(fabs(r[q]));
+        const unsigned long long b = __double_as_longlong(fabs((double)r[q]));
         m = m > b ? m : b;
     }
     m = wave_max_u64(m);
@@ -93,15 +105,16 @@ __global__ __launch_bounds__(256) void reduce_r(const double *r, int64_t n, unsi
 
 // First argmax in the reference's loop order (i, j, k, atom, dir): among the
 // dofs equal to the maximum, the smallest key ((i*ny + j)*nz + k)*96 + q.
-// r is in the material's lattice layout (IonLayout).
-__global__ __launch_bounds__(256) void argmax_r(const double *r, const uint32_t *cells, int64_t nslot,
-                                                IonLayout lay, int ny, int nz, int nxp, unsigned long long *out) {
+// r is in the material's lattice layout (IonLayout), values as reduce_r reads them.
+template <typename T>
+__global__ __launch_bounds__(256) void argmax_r(const T *r, const uint32_t *cells, int64_t nslot, IonLayout lay,
+                                                int ny, int nz, int nxp, unsigned long long *out) {
     const unsigned long long target = out[2];
     const int64_t total = (nslot + 63) / 64 * lay.tile_r();
     unsigned long long best = ~0ull;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
-        if ((unsigned long long)__double_as_longlong(fabs(r[q])) != target) continue;
+        if ((unsigned long long)__double_as_longlong(fabs((double)r[q])) != target) continue;
         int64_t s;
         int dof;
         if (!lay.r_decode(q, nslot, &s, &dof)) continue;
@@ -208,13 +221,22 @@ void launch_reduce_e(const real *Ex, const real *Ey, const real *Ez, int nx, int
     hipLaunchKernelGGL(reduce_e<real>, dim3(2048), dim3(256), 0, s, Ex, Ey, Ez, nx, ny, nz, nxp, out);
 }
 
-void launch_reduce_r(const double *r, int64_t n, unsigned long long *out, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(reduce_r, dim3(1024), dim3(256), 0, s, r, n, out);
+// r: double values, or float words when f32 (compensated state).
+void launch_reduce_r(const void *r, bool f32, int64_t n, unsigned long long *out, hipStream_t s) {
+    if (n <= 0) return;
+    if (f32) hipLaunchKernelGGL(reduce_r<float>, dim3(1024), dim3(256), 0, s, static_cast<const float *>(r), n, out);
+    else hipLaunchKernelGGL(reduce_r<double>, dim3(1024), dim3(256), 0, s, static_cast<const double *>(r), n, out);
 }
 
-void launch_argmax_r(const double *r, const uint32_t *cells, int64_t nslot, IonLayout lay, int ny, int nz, int nxp,
-                     unsigned long long *out, hipStream_t s) {
-    if (nslot > 0) hipLaunchKernelGGL(argmax_r, dim3(1024), dim3(256), 0, s, r, cells, nslot, lay, ny, nz, nxp, out);
+void launch_argmax_r(const void *r, bool f32, const uint32_t *cells, int64_t nslot, IonLayout lay, int ny, int nz,
+                     int nxp, unsigned long long *out, hipStream_t s) {
+    if (nslot <= 0) return;
+    if (f32)
+        hipLaunchKernelGGL(argmax_r<float>, dim3(1024), dim3(256), 0, s, static_cast<const float *>(r), cells, nslot,
+                           lay, ny, nz, nxp, out);
+    else
+        hipLaunchKernelGGL(argmax_r<double>, dim3(1024), dim3(256), 0, s, static_cast<const double *>(r), cells,
+                           nslot, lay, ny, nz, nxp, out);
 }
 
 template <typename real>

@@ -205,17 +205,41 @@ __host__ __device__ constexpr int mfma_wave_lds(int na) {
                                                                          : (3 * na + 15) / 16 * 16 * kRowA;
 }
 
-template <int NA, int WPB>
+// Compensated FP32 state: the four FP32 words and four compensation bytes of
+// dof group j of a lane (IonLayout w = 4, group j at base + 256 j).
+__device__ __forceinline__ f32x4 ldquad(const float *base, int j) {
+    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * 256));
+}
+__device__ __forceinline__ uint32_t ldbytes(const int8_t *base, int j) {
+    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(base + (int64_t)j * 256));
+}
+__device__ __forceinline__ void stquad(f32x4 v, float *base, int j) {
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * 256));
+}
+__device__ __forceinline__ void stbytes(uint32_t v, int8_t *base, int j) {
+    __builtin_nontemporal_store(v, reinterpret_cast<uint32_t *>(base + (int64_t)j * 256));
+}
+
+// Byte h of a packed dword as a signed value, and its replacement.
+__device__ __forceinline__ int byte_of(uint32_t w, int h) { return (int)(int8_t)(w >> (8 * h)); }
+__device__ __forceinline__ uint32_t with_byte(uint32_t w, int h, int b) {
+    return (w & ~(0xffu << (8 * h))) | ((uint32_t)(b & 0xff) << (8 * h));
+}
+
+template <int NA, int WPB, bool PK>
 __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const float *__restrict__ aop,
-                                               const float *__restrict__ qop, double *__restrict__ rr,
-                                               double *__restrict__ vv, const int64_t blk) {
+                                               const float *__restrict__ qop, void *__restrict__ rr_,
+                                               void *__restrict__ vv_, const int64_t blk) {
     constexpr bool exact = NA != kMaxAtom;
     constexpr int N3X = 3 * NA;
     constexpr int NKX = (N3X + 3) / 4, NAX = (N3X + 15) / 16;
     constexpr int WL = mfma_wave_lds(NA);
+    constexpr int W = PK ? 4 : 2;  // dofs per lane vector (IonLayout w)
+    constexpr int NVX = (N3X + W - 1) / W;
     const int natom = exact ? NA : a.natom;
     const int N3 = 3 * natom;
     const int NK = exact ? NKX : (N3 + 3) / 4, NMA = exact ? NAX : (N3 + 15) / 16;
+    const int NV = (N3 + W - 1) / W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t tile = blk * WPB + wave;
@@ -230,16 +254,24 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     }
     if (tile >= ntile) return;  // wave-uniform
     float *buf = lds_a + wave * WL;
-    const int NP = (N3 + 1) / 2;
-    constexpr int NPX = (N3X + 1) / 2;
-    const int64_t tb = tile * 128 * (int64_t)NP + 2 * lane;  // dof pair p of this lane's slot at tb + 128 p
-    double r[2 * NPX];
+    // dof group j of this lane's slot at tb + 64 W j (elements; bytes for the compensation array)
+    const int64_t tb = tile * 64 * W * (int64_t)NV + W * lane;
+    // r^n: FP64 values (FP64 state), or FP32 words + compensation dwords
+    double r[PK ? 1 : W * NVX];
+    float rw[PK ? W * NVX : 1];
+    uint32_t rc[PK ? NVX : 1];
 #pragma unroll
-    for (int p = 0; p < NPX; ++p)
-        if (p < NP) {
-            const f64x2 v2 = ldpair(rr + tb, p);
-            r[2 * p] = v2.x;
-            r[2 * p + 1] = v2.y;
+    for (int j = 0; j < NVX; ++j)
+        if (j < NV) {
+            if constexpr (PK) {
+                const f32x4 h4 = ldquad(static_cast<const float *>(rr_) + tb, j);
+                for (int h = 0; h < 4; ++h) rw[4 * j + h] = h4[h];
+                rc[j] = ldbytes(a.rl + tb, j);
+            } else {
+                const f64x2 v2 = ldpair(static_cast<const double *>(rr_) + tb, j);
+                r[2 * j] = v2.x;
+                r[2 * j + 1] = v2.y;
+            }
         }
     const int64_t slot = tile * 64 + lane;
     const bool live = slot < a.nslot;
@@ -256,12 +288,15 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
             Eold[2] = a.Ez[cell];
         }
     }
-    // 1. r^n in FP32 as LDS rows (rows 3N .. 4 nk of the last k-tile zero), P_ion
+    // 1. r^n in FP32 as LDS rows (rows 3N .. 4 nk of the last k-tile zero), P_ion.
+    //    The FP32 word of compensated state is fl32 of its value.
     float P[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < N3X; ++q) {
         if (q >= N3) break;
-        const float rf = (float)r[q];
+        float rf;
+        if constexpr (PK) rf = rw[q];
+        else rf = (float)r[q];
         buf[q * kRowB + lane] = rf;
         const int at = q / 3, d = q % 3;
         P[0] += sld(qop, at * 9 + 0 + d) * rf;
@@ -322,25 +357,56 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     __builtin_amdgcn_wave_barrier();
     const double dt = a.dt, hdt2 = a.half_dt2;
     const float *qm = qop + 9 * natom;
+    // FP64 integration r += v dt + a dt^2/2, v += a dt; dofs past 3N (padding) keep their zeros
 #pragma unroll
-    for (int p = 0; p < NPX; ++p) {
-        if (p >= NP) break;
-        f64x2 v2 = ldpair(vv + tb, p);
-        f64x2 r2 = {r[2 * p], r[2 * p + 1]};
+    for (int j = 0; j < NVX; ++j) {
+        if (j >= NV) break;
+        if constexpr (PK) {
+            float *rh = static_cast<float *>(rr_) + tb, *vh = static_cast<float *>(vv_) + tb;
+            f32x4 v4 = ldquad(vh, j), r4;
+            uint32_t vc = ldbytes(a.vl + tb, j), rcn = rc[j];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int q = 2 * p + h;
-            if (q >= N3) break;  // padding dof of an odd 3N stays 0
-            const int at = q / 3, d = q % 3;
-            const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] +
-                             sld(qm, at * 9 + 3 + d) * Ec[1] + sld(qm, at * 9 + 6 + d) * Ec[2];
-            const double am = (double)af;
-            const double vq = v2[h];
-            r2[h] = r[q] + (vq * dt + am * hdt2);
-            v2[h] = vq + am * dt;
+            for (int h = 0; h < 4; ++h) {
+                const int q = 4 * j + h;
+                r4[h] = rw[q];
+                if (q >= N3) continue;
+                const int at = q / 3, d = q % 3;
+                const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] +
+                                 sld(qm, at * 9 + 3 + d) * Ec[1] + sld(qm, at * 9 + 6 + d) * Ec[2];
+                const double am = (double)af;
+                const double rq = pk_unpack(rw[q], byte_of(rcn, h)), vq = pk_unpack(v4[h], byte_of(vc, h));
+                float hw;
+                int lo;
+                pk_pack(rq + (vq * dt + am * hdt2), &hw, &lo);
+                r4[h] = hw;
+                rcn = with_byte(rcn, h, lo);
+                pk_pack(vq + am * dt, &hw, &lo);
+                v4[h] = hw;
+                vc = with_byte(vc, h, lo);
+            }
+            stquad(r4, rh, j);
+            stbytes(rcn, a.rl + tb, j);
+            stquad(v4, vh, j);
+            stbytes(vc, a.vl + tb, j);
+        } else {
+            double *rd = static_cast<double *>(rr_) + tb, *vd = static_cast<double *>(vv_) + tb;
+            f64x2 v2 = ldpair(vd, j);
+            f64x2 r2 = {r[2 * j], r[2 * j + 1]};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = 2 * j + h;
+                if (q >= N3) break;  // padding dof of an odd 3N stays 0
+                const int at = q / 3, d = q % 3;
+                const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] +
+                                 sld(qm, at * 9 + 3 + d) * Ec[1] + sld(qm, at * 9 + 6 + d) * Ec[2];
+                const double am = (double)af;
+                const double vq = v2[h];
+                r2[h] = r[q] + (vq * dt + am * hdt2);
+                v2[h] = vq + am * dt;
+            }
+            stpair(r2, rd, j);
+            stpair(v2, vd, j);
         }
-        stpair(r2, rr + tb, p);
-        stpair(v2, vv + tb, p);
     }
 }
 
@@ -357,11 +423,11 @@ __global__ __launch_bounds__(256) void ion_update_f64(SlotArgs<double> a, const 
     ion_f64_slot<NA>(a, dmm, qezv, qezm, rr, vv, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
-template <int NA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ion_update_mfma(SlotArgs<float> a, const float *__restrict__ aop,
-                                                       const float *__restrict__ qop, double *__restrict__ rr,
-                                                       double *__restrict__ vv) {
-    ion_mfma_block<NA, NA == kMaxAtom ? 1 : 4>(a, aop, qop, rr, vv, blockIdx.x);
+template <int NA, bool PK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ion_update_mfma(
+    SlotArgs<float> a, const float *__restrict__ aop, const float *__restrict__ qop, void *__restrict__ rr,
+    void *__restrict__ vv) {
+    ion_mfma_block<NA, NA == kMaxAtom ? 1 : 4, PK>(a, aop, qop, rr, vv, blockIdx.x);
 }
 
 // Batched launches over the slot work of several domains: work-group g is
@@ -377,14 +443,16 @@ template <int NA>
 __global__ __launch_bounds__(256) void ion_update_f64_batch(BatchLaunch<SlotArgs<double>> b) {
     const int rid = (int)sld(b.map, blockIdx.x);
     const SlotArgs<double> a = batch_args(b, rid);
-    ion_f64_slot<NA>(a, a.dmm, a.qezv, a.qezm, a.r, a.v, (int64_t)(blockIdx.x - sld(b.start, rid)) * 256 + threadIdx.x);
+    ion_f64_slot<NA>(a, a.dmm, a.qezv, a.qezm, static_cast<double *>(a.r), static_cast<double *>(a.v),
+                     (int64_t)(blockIdx.x - sld(b.start, rid)) * 256 + threadIdx.x);
 }
 
-template <int NA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ion_update_mfma_batch(BatchLaunch<SlotArgs<float>> b) {
+template <int NA, bool PK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ion_update_mfma_batch(
+    BatchLaunch<SlotArgs<float>> b) {
     const int rid = (int)sld(b.map, blockIdx.x);
     const SlotArgs<float> a = batch_args(b, rid);
-    ion_mfma_block<NA, NA == kMaxAtom ? 1 : 4>(a, a.aop, a.qop, a.r, a.v, blockIdx.x - sld(b.start, rid));
+    ion_mfma_block<NA, NA == kMaxAtom ? 1 : 4, PK>(a, a.aop, a.qop, a.r, a.v, blockIdx.x - sld(b.start, rid));
 }
 
 template <typename real>
@@ -400,10 +468,10 @@ template <>
 void launch_ion<double>(const SlotArgs<double> &a, hipStream_t st) {
     const unsigned nb = (unsigned)((a.nslot + 255) / 256);
     if (!nb) return;
+    double *r = static_cast<double *>(a.r), *v = static_cast<double *>(a.v);
 #define PHPFDTD_ION_CASE(NN)                                                                              \
     case NN:                                                                                              \
-        hipLaunchKernelGGL(ion_update_f64<NN>, dim3(nb), dim3(256), 0, st, a, a.dmm, a.qezv, a.qezm, a.r, \
-                           a.v);                                                                          \
+        hipLaunchKernelGGL(ion_update_f64<NN>, dim3(nb), dim3(256), 0, st, a, a.dmm, a.qezv, a.qezm, r, v); \
         return;
     switch (a.natom) {
         PHPFDTD_ION_CASE(1)
@@ -415,7 +483,7 @@ void launch_ion<double>(const SlotArgs<double> &a, hipStream_t st) {
         PHPFDTD_ION_CASE(12)
         PHPFDTD_ION_CASE(16)
         default:
-            hipLaunchKernelGGL(ion_update_f64<0>, dim3(nb), dim3(256), 0, st, a, a.dmm, a.qezv, a.qezm, a.r, a.v);
+            hipLaunchKernelGGL(ion_update_f64<0>, dim3(nb), dim3(256), 0, st, a, a.dmm, a.qezv, a.qezm, r, v);
     }
 #undef PHPFDTD_ION_CASE
 }
@@ -434,14 +502,16 @@ template <>
 void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
     const int64_t ntile = (a.nslot + 63) / 64;
     if (!ntile) return;
+    const bool pk = a.rl != nullptr;  // compensated FP32 state
     // the host lays out aop/qop for the instantiation mfma_inst(natom)
     switch (mfma_inst(a.natom)) {
-#define PHPFDTD_MFMA_CASE(NN)                                                                  \
-    case NN: {                                                                                 \
-        const int wpb = mfma_threads(NN) / 64;                                                 \
-        hipLaunchKernelGGL(ion_update_mfma<NN>, dim3((unsigned)((ntile + wpb - 1) / wpb)),     \
-                           dim3(mfma_threads(NN)), mfma_lds_bytes(NN), st, a, a.aop, a.qop, a.r, a.v); \
-        return;                                                                                \
+#define PHPFDTD_MFMA_CASE(NN)                                                                               \
+    case NN: {                                                                                              \
+        const int wpb = mfma_threads(NN) / 64;                                                              \
+        const dim3 g((unsigned)((ntile + wpb - 1) / wpb)), blk(mfma_threads(NN));                           \
+        if (pk) hipLaunchKernelGGL((ion_update_mfma<NN, true>), g, blk, mfma_lds_bytes(NN), st, a, a.aop, a.qop, a.r, a.v); \
+        else hipLaunchKernelGGL((ion_update_mfma<NN, false>), g, blk, mfma_lds_bytes(NN), st, a, a.aop, a.qop, a.r, a.v); \
+        return;                                                                                             \
     }
         PHPFDTD_MFMA_CASE(1)
         PHPFDTD_MFMA_CASE(2)
@@ -471,7 +541,7 @@ int64_t slot_blocks(const SlotArgs<real> &a) {
 template <typename real>
 int slot_kind(const SlotArgs<real> &a) {
     if (a.natom == 0) return 0;
-    if (sizeof(real) == 4) return mfma_inst(a.natom);
+    if (sizeof(real) == 4) return mfma_inst(a.natom) + (a.rl ? kPackedKind : 0);
     switch (a.natom) {
         case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return a.natom;
         default: return kMaxAtom;
@@ -502,15 +572,21 @@ void launch_slot_batch<double>(const BatchLaunch<SlotArgs<double>> &b, int kind,
     }
 }
 
+// kind: slot_kind; the lattice kernels of a compensated-state batch are the
+// PK instantiations (kind + kPackedKind).
 template <>
 void launch_slot_batch<float>(const BatchLaunch<SlotArgs<float>> &b, int kind, hipStream_t st) {
     if (b.total <= 0) return;
     const dim3 g(b.total), blk(256);
     switch (kind) {
         case 0: hipLaunchKernelGGL(ade_update_batch<float>, g, blk, 0, st, b); return;
-#define PHPFDTD_MFMA_CASE(NN) \
-    case NN:                  \
-        hipLaunchKernelGGL(ion_update_mfma_batch<NN>, g, dim3(mfma_threads(NN)), mfma_lds_bytes(NN), st, b); return;
+#define PHPFDTD_MFMA_CASE(NN)                                                                                      \
+    case NN:                                                                                                       \
+        hipLaunchKernelGGL((ion_update_mfma_batch<NN, false>), g, dim3(mfma_threads(NN)), mfma_lds_bytes(NN), st, b); \
+        return;                                                                                                    \
+    case NN + kPackedKind:                                                                                         \
+        hipLaunchKernelGGL((ion_update_mfma_batch<NN, true>), g, dim3(mfma_threads(NN)), mfma_lds_bytes(NN), st, b);  \
+        return;
         PHPFDTD_MFMA_CASE(1)
         PHPFDTD_MFMA_CASE(2)
         PHPFDTD_MFMA_CASE(3)

@@ -39,8 +39,8 @@ template <typename T> void launch_fill(T *, T, int64_t, hipStream_t);
 template <typename real>
 void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
                      hipStream_t);
-void launch_reduce_r(const double *, int64_t, unsigned long long *, hipStream_t);
-void launch_argmax_r(const double *, const uint32_t *, int64_t, IonLayout, int, int, int, unsigned long long *,
+void launch_reduce_r(const void *, bool, int64_t, unsigned long long *, hipStream_t);
+void launch_argmax_r(const void *, bool, const uint32_t *, int64_t, IonLayout, int, int, int, unsigned long long *,
                      hipStream_t);
 template <typename real>
 void launch_slice(const real *, const real *, const real *, const real *, int, char, int, int, int, int, int,
@@ -73,11 +73,12 @@ struct Special {
     int id = 0;
     MatPre pre{};
     std::vector<uint32_t> cells;  // x-fastest field indices, ascending
-    IonLayout lay{};              // lattice state layout r[tile][dof][64]
+    IonLayout lay{};              // lattice state layout r[tile][dof / w][64][w]
     std::vector<double> qezv, qezm, dmm;
     uint32_t *d_cells = nullptr;
     void *d_P = nullptr, *d_Po = nullptr;
-    double *d_r = nullptr, *d_v = nullptr;
+    void *d_r = nullptr, *d_v = nullptr;        // FP64 values, or FP32 words of compensated state
+    int8_t *d_rl = nullptr, *d_vl = nullptr;    // compensation bytes (compensated state only)
     double *d_dmm = nullptr, *d_qezv = nullptr, *d_qezm = nullptr;
     float *d_aop = nullptr, *d_qop = nullptr;  // MFMA operand tables (precision form)
     SlotArgs<double> sa64{};                    // argument block of its slot kernel (current plan)
@@ -109,7 +110,8 @@ struct Layout {
 
 struct phpfdtd_ctx {
     phpfdtd_grid g{};
-    int prec = 0;
+    int prec = 0;     // PHPFDTD_FP64 or PHPFDTD_FP32 (the field form)
+    bool pk = false;  // lattice r, v in compensated FP32 (PHPFDTD_FP32), else FP64
     size_t rs = 8;
     int nx = 0, ny = 0, nz = 0, nxp = 0, L = 0;
     size_t plane = 0, ncellp = 0;
@@ -141,7 +143,8 @@ struct phpfdtd_ctx {
     int64_t ring_rows = 0, mon_read = 0;
     bool mon_on = false;
     int64_t mon_c = -1, mon_ridx = -1, mon_vidx = -1;
-    const double *mon_r = nullptr, *mon_v = nullptr;
+    const void *mon_r = nullptr, *mon_v = nullptr;
+    const int8_t *mon_rl = nullptr, *mon_vl = nullptr;
     unsigned long long *d_diag = nullptr;
     float *d_slice = nullptr;
     // near-field DFT of one plane
@@ -224,9 +227,10 @@ struct phpfdtd_ctx {
 
 static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nslot,
                           const std::vector<int> &natom, const std::vector<int> &ade) {
-    const bool mfma = g.precision == PHPFDTD_FP32;
+    const bool mfma = g.precision != PHPFDTD_FP64;
+    const bool pk = g.precision == PHPFDTD_FP32;  // compensated FP32 lattice state, 5 bytes per value
     Layout l;
-    const size_t rs = g.precision == PHPFDTD_FP32 ? 4 : 8;
+    const size_t rs = mfma ? 4 : 8;
     l.nxp = (g.nx + 31) / 32 * 32;
     l.ncellp = (size_t)l.nxp * g.ny * g.nz;
     l.field = l.ncellp * rs;
@@ -243,8 +247,8 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
         l.slots += n * 4;
         if (ade[m]) l.slots += 2 * 3 * n * rs;
         if (natom[m]) {
-            const IonLayout il = ion_layout(natom[m]);
-            l.slots += (n + 63) / 64 * (size_t)(il.tile_r() + il.tile_v()) * 8 + 9 * natom[m] * natom[m] * 8 +
+            const IonLayout il = ion_layout(natom[m], pk);
+            l.slots += (n + 63) / 64 * (size_t)(il.tile_r() + il.tile_v()) * (pk ? 5 : 8) + 9 * natom[m] * natom[m] * 8 +
                        18 * natom[m] * 8;
             if (mfma)
                 l.slots += ((size_t)mfma_nma(natom[m]) * mfma_nk(natom[m]) * 64 + 18 * natom[m]) * sizeof(float);
@@ -425,6 +429,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         sa.Po = (real *)sp.d_Po;
         sa.natom = sp.pre.natom;
         sa.r = sp.d_r; sa.v = sp.d_v;
+        sa.rl = sp.d_rl; sa.vl = sp.d_vl;
         sa.lay = sp.lay;
         sa.dmm = sp.d_dmm; sa.qezv = sp.d_qezv; sa.qezm = sp.d_qezm;
         sa.aop = sp.d_aop; sa.qop = sp.d_qop;
@@ -436,7 +441,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
         if (sp.pre.natom > 0) {
             const double N3 = 3.0 * sp.pre.natom;
             snprintf(nm, sizeof(nm), "ion_update[m%d,N%d]", sp.id, sp.pre.natom);
-            k.bytes = n * (6 * w + 4 + 4 * N3 * 8 + (sp.pre.ade ? 12 * w : 0));
+            // r, v read and written: 8 bytes per value, 5 for compensated FP32
+            k.bytes = n * (6 * w + 4 + 4 * N3 * (x->pk ? 5 : 8) + (sp.pre.ade ? 12 * w : 0));
             k.flops = n * (2 * N3 * N3 + 12 * N3 + 40);
             k.stage = K_ION;
             k.fn = [sa](hipStream_t s) { launch_ion<real>(sa, s); };
@@ -470,6 +476,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
     for (int q = 0; q < 9; ++q) se.F[q] = x->F[q];
     se.mon_r = x->mon_r;
     se.mon_v = x->mon_v;
+    se.mon_rl = x->mon_rl;
+    se.mon_vl = x->mon_vl;
     se.mon_ridx = x->mon_ridx;
     se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
@@ -614,7 +622,8 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     phpfdtd_ctx *x = new phpfdtd_ctx();
     std::unique_ptr<phpfdtd_ctx> guard(x);
     x->g = *g;
-    x->prec = g->precision;
+    x->prec = g->precision == PHPFDTD_FP64 ? PHPFDTD_FP64 : PHPFDTD_FP32;
+    x->pk = g->precision == PHPFDTD_FP32;
     x->rs = x->prec == PHPFDTD_FP32 ? 4 : 8;
     x->nx = g->nx; x->ny = g->ny; x->nz = g->nz; x->L = g->pml_thickness;
     x->nmat = nmat;
@@ -726,10 +735,15 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
             if ((rc = x->alloc(&sp.d_Po, 3 * n * x->rs))) return rc;
         }
         if (sp.pre.natom > 0) {
-            sp.lay = ion_layout(sp.pre.natom);
+            sp.lay = ion_layout(sp.pre.natom, x->pk);
             const size_t ntile = (n + 63) / 64;  // 64-slot tiles
-            if ((rc = x->alloc((void **)&sp.d_r, ntile * sp.lay.tile_r() * 8))) return rc;
-            if ((rc = x->alloc((void **)&sp.d_v, ntile * sp.lay.tile_v() * 8))) return rc;
+            const size_t es = x->pk ? 4 : 8;
+            if ((rc = x->alloc(&sp.d_r, ntile * sp.lay.tile_r() * es))) return rc;
+            if ((rc = x->alloc(&sp.d_v, ntile * sp.lay.tile_v() * es))) return rc;
+            if (x->pk) {
+                if ((rc = x->alloc((void **)&sp.d_rl, ntile * sp.lay.tile_r()))) return rc;
+                if ((rc = x->alloc((void **)&sp.d_vl, ntile * sp.lay.tile_v()))) return rc;
+            }
             if ((rc = x->upload((void **)&sp.d_dmm, sp.dmm))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezv, sp.qezv))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezm, sp.qezm))) return rc;
@@ -821,6 +835,7 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
     x->mon_c = zf < x->nz ? (int64_t)x->fidx(mx, my, zf) : -1;
     x->mon_ridx = x->mon_vidx = -1;
     x->mon_r = x->mon_v = nullptr;
+    x->mon_rl = x->mon_vl = nullptr;
     const int m = x->medium[x->fidx(mx, my, mz)];
     const int sp = x->special_of[m];
     if (sp >= 0 && atom < x->pre[m].natom) {
@@ -830,6 +845,8 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
         x->mon_vidx = x->specials[sp].lay.v_index(slot, atom * 3 + dir);
         x->mon_r = x->specials[sp].d_r;
         x->mon_v = x->specials[sp].d_v;
+        x->mon_rl = x->specials[sp].d_rl;
+        x->mon_vl = x->specials[sp].d_vl;
     }
     touch(x);
     return 0;
@@ -984,10 +1001,11 @@ int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
                                x->d_diag, x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
-            launch_reduce_r(sp.d_r, (int64_t)(sp.cells.size() + 63) / 64 * sp.lay.tile_r(), x->d_diag, x->stream);
+            launch_reduce_r(sp.d_r, x->pk, (int64_t)(sp.cells.size() + 63) / 64 * sp.lay.tile_r(), x->d_diag,
+                            x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
-            launch_argmax_r(sp.d_r, sp.d_cells, (int64_t)sp.cells.size(), sp.lay, x->ny, x->nz, x->nxp,
+            launch_argmax_r(sp.d_r, x->pk, sp.d_cells, (int64_t)sp.cells.size(), sp.lay, x->ny, x->nz, x->nxp,
                             x->d_diag, x->stream);
     unsigned long long res[4];
     HIPCHK(hipMemcpyAsync(res, x->d_diag, sizeof(res), hipMemcpyDeviceToHost, x->stream));
@@ -1091,8 +1109,21 @@ int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double 
     HIPCHK(hipStreamSynchronize(x->stream));
     (void)n;
     for (int q = 0; q < n3; ++q) {
-        if (r) HIPCHK(hipMemcpy(r + q, s.d_r + s.lay.r_index(slot, q), 8, hipMemcpyDeviceToHost));
-        if (v) HIPCHK(hipMemcpy(v + q, s.d_v + s.lay.v_index(slot, q), 8, hipMemcpyDeviceToHost));
+        for (int k = 0; k < 2; ++k) {
+            double *out = k == 0 ? r : v;
+            if (!out) continue;
+            const size_t at = (size_t)(k == 0 ? s.lay.r_index(slot, q) : s.lay.v_index(slot, q));
+            const char *dev = (const char *)(k == 0 ? s.d_r : s.d_v);
+            if (x->pk) {
+                float w;
+                int8_t b;
+                HIPCHK(hipMemcpy(&w, dev + at * 4, 4, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&b, (k == 0 ? s.d_rl : s.d_vl) + at, 1, hipMemcpyDeviceToHost));
+                out[q] = pk_unpack(w, b);
+            } else {
+                HIPCHK(hipMemcpy(out + q, dev + at * 8, 8, hipMemcpyDeviceToHost));
+            }
+        }
     }
     return x->pre[m].natom;
 }
@@ -1160,12 +1191,14 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     const size_t n = s.cells.size();
     int width;
     void *dev;
+    int8_t *devl = nullptr;  // compensation bytes of compensated lattice state
     size_t es;
     if (kind < 2) {
         if (!s.pre.natom) return fail(PHPFDTD_EINVAL, "material %d has no lattice", mat);
         width = 3 * s.pre.natom;
-        dev = kind == 0 ? (void *)s.d_r : (void *)s.d_v;
-        es = 8;
+        dev = kind == 0 ? s.d_r : s.d_v;
+        devl = kind == 0 ? s.d_rl : s.d_vl;
+        es = x->pk ? 4 : 8;
     } else {
         if (!s.pre.ade) return fail(PHPFDTD_EINVAL, "material %d has no ADE state", mat);
         width = 3;
@@ -1176,15 +1209,29 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     const bool tiled = kind < 2;  // lattice r, v in 64-slot tiles; ADE P, Po as [3][nslot]
     const size_t nelem = tiled ? (n + 63) / 64 * (size_t)(kind == 0 ? s.lay.tile_r() : s.lay.tile_v()) : n * width;
     std::vector<unsigned char> h(nelem * es);
+    std::vector<int8_t> hl(devl ? nelem : 0);
     const std::vector<int64_t> ord = file_order(x, s);
-    if (!write || tiled) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
+    if (!write || tiled) {
+        HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
+        if (devl) HIPCHK(hipMemcpy(hl.data(), devl, hl.size(), hipMemcpyDeviceToHost));
+    }
     for (size_t q = 0; q < n; ++q) {
         const int64_t slot = ord[q];
         for (int d = 0; d < width; ++d) {
             const size_t hi = tiled ? (size_t)(kind == 0 ? s.lay.r_index(slot, d) : s.lay.v_index(slot, d))
                                     : (size_t)d * n + slot;
             double &u = buf[q * width + d];
-            if (es == 8) {
+            if (devl) {  // compensated FP32: written values are rounded to 32 significant bits
+                float w;
+                int b;
+                if (write) {
+                    pk_pack(u, &w, &b);
+                    ((float *)h.data())[hi] = w;
+                    hl[hi] = (int8_t)b;
+                } else {
+                    u = pk_unpack(((float *)h.data())[hi], hl[hi]);
+                }
+            } else if (es == 8) {
                 if (write) ((double *)h.data())[hi] = u;
                 else u = ((double *)h.data())[hi];
             } else {
@@ -1193,7 +1240,10 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
             }
         }
     }
-    if (write) HIPCHK(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
+    if (write) {
+        HIPCHK(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
+        if (devl) HIPCHK(hipMemcpy(devl, hl.data(), hl.size(), hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -1287,7 +1337,8 @@ int phpfdtd_struct_sizes(int *sizes, int n) {
 // tables; two pinned staging buffers let the host build the next tables while
 // the previous upload is still pending.
 struct phpfdtd_batch {
-    int prec = 0;
+    int prec = 0;  // field form of the members (PHPFDTD_FP64 / PHPFDTD_FP32)
+    int form = 0;  // the members' precision value (PHPFDTD_FP64 / FP32 / FP32_RV64)
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
@@ -1472,10 +1523,12 @@ static void batch_step_t(phpfdtd_batch *b) {
 extern "C" {
 
 int phpfdtd_batch_create(int precision, phpfdtd_batch **out) {
-    if (!out || (precision != PHPFDTD_FP64 && precision != PHPFDTD_FP32)) return fail(PHPFDTD_EINVAL, "bad arguments");
+    if (!out || (precision != PHPFDTD_FP64 && precision != PHPFDTD_FP32 && precision != PHPFDTD_FP32_RV64))
+        return fail(PHPFDTD_EINVAL, "bad arguments");
     *out = nullptr;
     std::unique_ptr<phpfdtd_batch> b(new phpfdtd_batch());
-    b->prec = precision;
+    b->prec = precision == PHPFDTD_FP64 ? PHPFDTD_FP64 : PHPFDTD_FP32;
+    b->form = precision;
     HIPCHK(hipGetDevice(&b->device));
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
@@ -1490,7 +1543,8 @@ int phpfdtd_batch_create(int precision, phpfdtd_batch **out) {
 int phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (!b || !x) return fail(PHPFDTD_EINVAL, "null argument");
     if (x->batch) return fail(PHPFDTD_ESTATE, "context is already in a batch");
-    if (x->prec != b->prec) return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->prec, b->prec);
+    if (x->g.precision != b->form)
+        return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->g.precision, b->form);
     if (x->device != b->device) return fail(PHPFDTD_EINVAL, "context is on device %d, batch on %d", x->device, b->device);
     if (x->yee_rows != 1) return fail(PHPFDTD_EINVAL, "batched launches use one y row per wavefront");
     HIPCHK(hipSetDevice(x->device));

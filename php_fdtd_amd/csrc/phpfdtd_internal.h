@@ -11,11 +11,12 @@
 //       x slabs [nz][ny][2L], y slabs [nz][2L][nxp], z slabs [2L][ny][nxp].
 //   * ADE and lattice state: per-material slot arrays, slot s of material m
 //     is the s-th cell of m in x-fastest order; ADE P[3][slot]; lattice r, v
-//     in 64-slot tiles r[tile][dof][64] (IonLayout).
+//     in 64-slot tiles r[tile][dof / w][64][w] (IonLayout).
 // FP64 host precompute (CPML b/c/kappa, eps^-1, ADE coefficients, rotations,
 // source values) is cast to the working type only when uploaded.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "../../include/phpfdtd.h"
@@ -25,6 +26,7 @@ namespace phpfdtd {
 constexpr int kMaxMat = 64;     // material ids are stored as uint8
 constexpr int kMaxSrc = 8;      // sources handled inside the D/E kernel
 constexpr int kMaxAtom = 32;    // lattice atoms per cell
+constexpr int kPackedKind = 64; // batch kind offset of compensated-state lattice kernels
 constexpr int kBlockY = 4;      // wavefronts per Yee work-group (64 x 4 threads)
 constexpr double kEpsInf = 20.0;  // EPS_INF of the reference ADE (fdtd.c:202)
 
@@ -92,33 +94,73 @@ inline int mfma_inst(int natom) {
     }
 }
 
-// Lattice state layout of one material (r and v, FP64, both forms): tiles of
-// 64 slots, dofs taken in pairs, r[tile][dof / 2][64][2] - a lane owns a slot
-// and reads the two dofs of a pair as one 16-byte vector, so every wavefront
-// access is one 1 KiB row.  An odd 3N is padded with one zero dof.
+// Lattice state layout of one material: tiles of 64 slots, the 3N dofs of a
+// slot taken in groups of w, r[tile][dof / w][64][w].  A lane owns a slot and
+// reads a group as one 16-byte vector, so every wavefront access is one 1 KiB
+// row:
+//   * FP64 state (plain form, and the precision form with FP64 r, v): w = 2;
+//   * compensated FP32 state (precision form): w = 4, FP32 words in one array
+//     and their int8 compensation bytes in a second array with the same
+//     indices (a lane's group of four bytes is one dword).
+// 3N is padded to a multiple of w with zero dofs.
 struct IonLayout {
     int n3;
-    __host__ __device__ int n3p() const { return (n3 + 1) & ~1; }
+    int w;  // dofs per lane vector: 2 (FP64) or 4 (compensated FP32)
+    __host__ __device__ int n3p() const { return (n3 + w - 1) / w * w; }
     __host__ __device__ int64_t tile_r() const { return 64LL * n3p(); }
     __host__ __device__ int64_t tile_v() const { return tile_r(); }
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
-        return (s >> 6) * tile_r() + (int64_t)(q >> 1) * 128 + (s & 63) * 2 + (q & 1);
+        return (s >> 6) * tile_r() + (int64_t)(q / w) * 64 * w + (s & 63) * w + q % w;
     }
     __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
     // slot and dof of r element idx; false for padding
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
         const int64_t t = idx / tile_r(), rem = idx % tile_r();
-        const int w = (int)(rem % 128);
-        *q = (int)(rem / 128) * 2 + (w & 1);
-        *s = t * 64 + w / 2;
+        const int u = (int)(rem % (64 * w));
+        *q = (int)(rem / (64 * w)) * w + u % w;
+        *s = t * 64 + u / w;
         return *q < n3 && *s < nslot;
     }
 };
 
-inline IonLayout ion_layout(int natom) {
+inline IonLayout ion_layout(int natom, bool packed = false) {
     IonLayout l;
     l.n3 = 3 * natom;
+    l.w = packed ? 4 : 2;
     return l;
+}
+
+// Compensated FP32 value (hi, lo): x = hi + lo * 2^(e - 158), e the biased
+// exponent of hi clamped to >= 1, so lo counts units of 2^-8 of hi's ulp and
+// |lo| <= 127 covers the rounding error of hi = fl32(x).  The stored value
+// carries 32 significant bits; pack and unpack are exact in FP64 apart from
+// that final rounding of x - hi to a multiple of 2^-8 ulp.
+__host__ __device__ inline int pk_exp(float hi) {
+    uint32_t b;
+    __builtin_memcpy(&b, &hi, 4);
+    const int e = (int)((b >> 23) & 0xffu);
+    return e > 1 ? e : 1;
+}
+// 2^k as a double, for the exponent range pk_* uses (-157 .. 157)
+__host__ __device__ inline double pk_pow2(int k) {
+    const uint64_t b = (uint64_t)(k + 1023) << 52;
+    double d;
+    __builtin_memcpy(&d, &b, 8);
+    return d;
+}
+__host__ __device__ inline double pk_unpack(float hi, int lo) {
+    return fma((double)lo, pk_pow2(pk_exp(hi) - 158), (double)hi);  // exact
+}
+__host__ __device__ inline void pk_pack(double x, float *hi, int *lo) {
+    const float h = (float)x;
+    // (x - h) in units of 2^-8 ulp(h), |.| <= 128, rounded to nearest even by
+    // adding 1.5 * 2^52: the integer lands in the low word of the sum
+    const double q = (x - (double)h) * pk_pow2(158 - pk_exp(h)) + 6755399441055744.0;
+    uint64_t b;
+    __builtin_memcpy(&b, &q, 8);
+    const int n = (int)(uint32_t)b;
+    *hi = h;
+    *lo = n < -127 ? -127 : n > 127 ? 127 : n;  // a non-finite x (diverged state) gives any code
 }
 
 // Lattice / ADE slot kernels: one material per launch.
@@ -137,7 +179,8 @@ struct SlotArgs {
     real *P, *Po;  // [3][nslot]
     // lattice (fdtd.c:3505-3528)
     int natom;
-    double *r, *v;           // FP64 state in both forms, layout `lay`
+    void *r, *v;             // FP64 state (double), or the FP32 words of compensated state
+    int8_t *rl, *vl;         // compensation bytes of compensated state, else null
     IonLayout lay;
     const double *qezv;      // [N][3][3]
     const double *qezm;      // [N][3][3]
@@ -158,7 +201,8 @@ struct StepEndArgs {
     int mon_on;
     int64_t mon_c;     // field index sampled (z+1 in reference mode), -1 = zero halo
     const void *F[9];  // field arrays (real)
-    const double *mon_r, *mon_v;  // lattice state of the monitor cell's material
+    const void *mon_r, *mon_v;    // lattice state of the monitor cell's material (double, or
+    const int8_t *mon_rl, *mon_vl;  // float words + compensation bytes when these are set)
     int64_t mon_ridx, mon_vidx;   // offsets of (slot, atom*3+dir) in r and v, -1 = no R/V
     double dt;
     int part;                     // 0 whole row, 1 D and H, 2 t, E, R, V + step advance

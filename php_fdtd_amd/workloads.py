@@ -324,10 +324,11 @@ def cell_counts(spec, med):
     return counts, pml
 
 
-def step_bytes(spec, med, word):
+def step_bytes(spec, med, word, rv_bytes=8):
     """Algorithmic HBM bytes of one time step: each state word read and written
     once.  C (18w + 1) + 8w * CPML incidences + 9w * ADE cells
-    + 96 N * lattice cells (r, v FP64 in both forms)."""
+    + 12 N b * lattice cells, b the bytes per stored r or v value (8 for FP64,
+    5 for compensated FP32): 96 N or 60 N."""
     counts, pml = cell_counts(spec, med)
     C = med.size
     b = C * (18 * word + 1) + 8 * word * pml
@@ -337,7 +338,7 @@ def step_bytes(spec, med, word):
         if mat.get("use_ade"):
             b += 9 * word * counts[m]
         if mat.get("natom", 0):
-            b += 96 * mat["natom"] * counts[m]
+            b += 12 * rv_bytes * mat["natom"] * counts[m]
     return float(b)
 
 

@@ -55,7 +55,7 @@ def alone(case, precision, steps):
     return snap
 
 
-@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32])
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32, _abi.FP32_RV64])
 def test_batch_members_match_alone(precision):
     cases = [golden_cases.load(n) for n in CASES]
     T = 100

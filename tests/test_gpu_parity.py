@@ -183,23 +183,69 @@ def small_lattice_case(natom):
 def test_lattice_atom_counts(natom):
     spec, med = small_lattice_case(natom)
     print("lattice natom", natom, flush=True)
-    sims = {p: Simulation(spec, med, p) for p in (_abi.FP64, _abi.FP32)}
+    forms = (_abi.FP64, _abi.FP32, _abi.FP32_RV64)
+    sims = {p: Simulation(spec, med, p) for p in forms}
     o = orc.Oracle(spec, med)
     randomize(spec, med, 99, sims[_abi.FP64], o)
-    randomize(spec, med, 99, sims[_abi.FP32], orc.Oracle(spec, med))
+    for p in forms[1:]:
+        randomize(spec, med, 99, sims[p], orc.Oracle(spec, med))
     for s in sims.values():
         s.run(5)
     o.step(5)
     comps = list(range(9))
     for comp in comps:
         assert rel_l2(sims[_abi.FP64].field(comp), o.field(comp)) <= RANDOM_TOL, comp
-        assert rel_l2(sims[_abi.FP32].field(comp), o.field(comp)) <= FP32_TOL, comp
+        for p in forms[1:]:
+            assert rel_l2(sims[p].field(comp), o.field(comp)) <= FP32_TOL, (p, comp)
     for kind in (0, 1):
         ref = o.state(4, kind)
         assert rel_l2(sims[_abi.FP64].state(4, kind), ref) <= RANDOM_TOL, kind
-        assert rel_l2(sims[_abi.FP32].state(4, kind), ref) <= FP32_TOL, kind
+        for p in forms[1:]:
+            assert rel_l2(sims[p].state(4, kind), ref) <= FP32_TOL, (p, kind)
     for s in sims.values():
         s.close()
+
+
+def test_compensated_state():
+    """Lattice r, v of the FP32 form are stored as an FP32 word plus an 8-bit
+    compensation term: any double written comes back within 2^-31 of itself
+    (2^-33 but for the clamped last code), values below the FP32 range within
+    2^-157, and storing what was read changes nothing."""
+    c = golden_cases.load("moo3")
+    spec = c["spec"]
+    sim = Simulation(spec, c["medium"], _abi.FP32)
+    rng = np.random.default_rng(11)
+    n = sim.count_cells(4)
+    for kind, scale in ((0, 1e-12), (1, 1e2)):
+        a = rng.standard_normal(n * 48) * scale * np.exp(rng.uniform(-20, 20, n * 48))
+        a[:5] = [0.0, -0.0, scale, -scale, 2.0 ** -130]
+        sim.set_state(4, kind, a)
+        b = sim.state(4, kind)
+        big = np.abs(a) >= 2.0 ** -126
+        assert np.all(np.abs(b[big] - a[big]) <= 2.0 ** -31 * np.abs(a[big])), kind
+        assert np.all(np.abs(b[~big] - a[~big]) <= 2.0 ** -157), kind
+        sim.set_state(4, kind, b)
+        np.testing.assert_array_equal(sim.state(4, kind), b)
+    sim.close()
+
+
+@pytest.mark.parametrize("name", ["hbn", "moo3"])
+def test_compensated_vs_fp64_state(name):
+    """The FP32 form with compensated FP32 r, v against the same form with
+    FP64 r, v over the whole golden run: the storage adds no error visible
+    beside the FP32 fields (monitor, full monitor and diagnostics agree to
+    1e-5 relative L2)."""
+    c = golden_cases.load(name)
+    print("compensated vs fp64 storage", name, flush=True)
+    res = {}
+    for p in (_abi.FP32, _abi.FP32_RV64):
+        rows, slices, full, d = run_case(c, p)
+        res[p] = (rows, full, d)
+    a, b = res[_abi.FP32], res[_abi.FP32_RV64]
+    assert rel_l2(a[0][:, 1:], b[0][:, 1:]) <= 1e-5
+    if a[1] is not None and len(a[1]):
+        assert rel_l2(a[1], b[1]) <= 1e-5
+    assert abs(a[2].max_r - b[2].max_r) <= 1e-5 * max(b[2].max_r, 1e-300)
 
 
 def test_graph_matches_stream_launches(monkeypatch):
