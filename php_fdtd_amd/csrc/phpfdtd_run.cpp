@@ -11,10 +11,11 @@
 // (fdtd.c:3102-3917).  Outputs at step t hold the fields after that step's
 // update, as the reference's do.
 //
-//   phpfdtd_run case.cfg [--fp32] [--precise]
+//   phpfdtd_run case.cfg [--fp32 | --fp32-rv64] [--precise]
 //
-// --fp32 selects the precision form; --precise writes the monitor files with
-// %.17e instead of the reference's %e.
+// --fp32 selects the precision form (ion r, v stored as compensated FP32),
+// --fp32-rv64 the precision form with r, v stored in FP64; --precise writes
+// the monitor files with %.17e instead of the reference's %e.
 #include <ctype.h>
 #include <math.h>
 #include <stdio.h>
@@ -537,17 +538,19 @@ int main(int argc, char **argv) {
     bool precise = false;
     for (int a = 1; a < argc; ++a) {
         if (!strcmp(argv[a], "--fp32")) precision = PHPFDTD_FP32;
+        else if (!strcmp(argv[a], "--fp32-rv64")) precision = PHPFDTD_FP32_RV64;
         else if (!strcmp(argv[a], "--precise")) precise = true;
         else cfg_path = argv[a];
     }
     if (!cfg_path) {
-        fprintf(stderr, "usage: %s case.cfg [--fp32] [--precise]\n", argv[0]);
+        fprintf(stderr, "usage: %s case.cfg [--fp32 | --fp32-rv64] [--precise]\n", argv[0]);
         return 2;
     }
     Cfg c = read_config(cfg_path);
     load_materials(c);
     printf("Grid %d x %d x %d, %d steps, dt = %.6e s, %s form\n", c.nx, c.ny, c.nz, c.tsteps, c.dt,
-           precision == PHPFDTD_FP32 ? "FP32 precision" : "FP64 plain");
+           precision == PHPFDTD_FP32 ? "FP32 precision" : precision == PHPFDTD_FP32_RV64 ? "FP32 precision (r, v FP64)"
+                                                                                    : "FP64 plain");
 
     // medium_map.bin: int32 nx, ny, nz, then ids with z fastest (fdtd.c:718-959)
     std::vector<int32_t> med((size_t)c.nx * c.ny * c.nz);
