@@ -28,6 +28,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+PROFILE = os.path.join(ROOT, "profiles", "cfg3_step_kernels.md")
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 METRIC = "Gcell-updates/s + HBM roofline fraction; solves/s, p50/p99, FP32 vs FP64"
 HBM_PEAK_GBS = 8000.0
@@ -35,6 +37,58 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def copy_ceiling(gib=4.0, iters=5):
+    """Measured streaming ceilings of this device (php_fdtd_amd/bwprobe): the
+    non-temporal copy and the two-buffer in-place update, GB/s."""
+    exe = os.path.join(ROOT, "php_fdtd_amd", "bwprobe")
+    out = {}
+    if not os.path.exists(exe):
+        return None
+    for v in ("nt", "ip"):
+        p = subprocess.run([exe, str(gib), str(iters), v], capture_output=True, text=True, timeout=120)
+        for ln in p.stdout.splitlines():
+            if ln.startswith("{"):
+                out[v] = json.loads(ln)["gbps"]
+    return {"copy_nt_gbs": out.get("nt"), "inplace_2buf_gbs": out.get("ip"), "buffer_gib": gib,
+            "source": "php_fdtd_amd/bwprobe %g %d nt|ip" % (gib, iters)}
+
+
+def profile_traffic(symbol):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-probe factors),
+    or None when the summary has no row for it."""
+    if not os.path.exists(PROFILE):
+        return None
+    in_pmc = False
+    for ln in open(PROFILE):
+        if ln.startswith("## "):
+            in_pmc = ln.startswith("## HBM traffic")
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if in_pmc and len(cells) == 7 and cells[0] == "`%s`" % symbol:
+            try:
+                launches, traffic = int(cells[1]), float(cells[4])
+            except ValueError:
+                return None
+            steps = 6  # the PMC runs step each form 1 + 2 + 3 times
+            per_step = max(1, launches // steps)
+            return traffic * 1e9 / per_step
+    return None
+
+
+def kernel_symbol(name, form):
+    """Device symbol of a plan kernel name of the given form."""
+    real = "double" if form == "fp64" else "float"
+    if name.startswith("ion_update"):
+        n = int(name.split(",N")[1].rstrip("]"))
+        if form == "fp64":
+            return "phpfdtd::ion_update_f64<%d>" % n
+        return "phpfdtd::ion_update_mfma<%d, %s>" % (n, "true" if form == "fp32" else "false")
+    base = name.split("[")[0]
+    if base in ("h_update", "de_update"):
+        return "phpfdtd::%s<%s, 1>" % (base, real)
+    return "phpfdtd::%s<%s>" % (base, real)
 
 
 def run_form(spec, med, precision, warmup, steps, prof_steps):
@@ -52,10 +106,22 @@ def run_form(spec, med, precision, warmup, steps, prof_steps):
     ms = sim.time_steps(steps)  # HIP events on the library stream, synced on both sides
     wall = time.perf_counter() - wall0
     prof = sim.profile(prof_steps)
+    # the same steps with output sampling: a z slice and the monitor rows read
+    # back every `interval` steps, as a reference run with output_interval does
+    interval, chunks = 10, max(1, steps // 10)
+    sim.sync(diag=False)
+    t0 = time.perf_counter()
+    for _ in range(chunks):
+        sim.run(interval)
+        sim.slice("Ez", "z", spec["sources"][0]["z"])
+        sim.monitor()
+    sim.sync(diag=False)
+    sampled_ms = (time.perf_counter() - t0) * 1e3 / (chunks * interval)
     rows = sim.monitor()
     d = sim.sync()
     sim.close()
-    return dict(ms=ms, wall_ms=wall * 1e3 / steps, prof=prof, rows=rows, max_e=d.max_e, max_r=d.max_r)
+    return dict(ms=ms, wall_ms=wall * 1e3 / steps, prof=prof, rows=rows, max_e=d.max_e, max_r=d.max_r,
+                sampled_ms=sampled_ms)
 
 
 def central(a):
@@ -246,6 +312,8 @@ def main():
     ap.add_argument("--accuracy", default="cfg1", help="config for the FP32-vs-FP64 accuracy run ('' = skip)")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving mix (0 = skip)")
     ap.add_argument("--serving-step-scale", type=float, default=0.05)
+    ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false",
+                    help="skip the measured copy-bandwidth ceilings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the domain is resident on a single MI355X and no "
@@ -257,7 +325,10 @@ def main():
     prof_steps = 3
     t0 = time.time()
     spec, med = workloads.CONFIGS[args.config]()
-    spec = workloads.with_steps(spec, args.warmup + args.steps + prof_steps)
+    spec = workloads.with_steps(spec, args.warmup + args.steps + prof_steps + max(1, args.steps // 10) * 10)
+    ceiling = copy_ceiling() if args.copy_probe else None
+    if ceiling:
+        log("copy ceilings: %s" % ceiling)
     log("workload %s: %s cells, built in %.1f s" % (spec["name"], med.size, time.time() - t0))
     cells = float(med.size)
     res = {}
@@ -267,7 +338,8 @@ def main():
         prec = precs[form]
         log("form %s" % form)
         res[form] = run_form(spec, med, prec, args.warmup, args.steps, prof_steps)
-        log("  %.3f ms/step (events), %.3f ms/step (wall)" % (res[form]["ms"], res[form]["wall_ms"]))
+        log("  %.3f ms/step (events), %.3f ms/step (wall), %.3f ms/step with a slice + monitor read every 10 steps" % (
+            res[form]["ms"], res[form]["wall_ms"], res[form]["sampled_ms"]))
         for k in res[form]["prof"]:
             log("    %-24s %9.3f ms  %8.1f GB/s" % (k["name"], k["ms"], k["bytes"] / k["ms"] / 1e6 if k["ms"] else 0))
 
@@ -277,6 +349,8 @@ def main():
     gcell = cells / (r["ms"] * 1e-3) / 1e9
     dom = max(r["prof"], key=lambda k: k["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    dom_sym = kernel_symbol(dom["name"], head)
+    traffic = profile_traffic(dom_sym)
     sb = workloads.step_bytes(spec, med, word, rv_bytes[head])
     rv_desc = {"fp32": "compensated f32 (f64 integration)", "fp32rv64": "f64", "fp64": "f64"}[head]
     out = {
@@ -298,13 +372,37 @@ def main():
                                                        "f32", rv_desc),
                    "cells": int(cells), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic) if traffic else None,
+                     "algorithmic_bytes": round(dom["bytes"]), "symbol": dom_sym,
+                     "traffic_source": "profiles/cfg3_step_kernels.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per "
+                                       "dispatch, copy-probe calibrated" if traffic else None},
         "step_roofline": {"bytes_per_step": sb, "achieved": round(sb / (r["ms"] * 1e-3) / 1e9, 1),
                           "frac": round(sb / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "model": "C(18w+1) + 8w*CPML incidences + 9w*ADE cells + 12Nb*lattice cells, b = %d "
                                    "bytes per stored r or v value" % rv_bytes[head]},
         "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r["prof"]},
+        "ms_per_step_with_sampling": round(r["sampled_ms"], 4),
     }
+    if ceiling:
+        out["measured_ceiling"] = dict(ceiling, dominant_frac_of_inplace=round(
+            achieved / ceiling["inplace_2buf_gbs"], 4) if ceiling.get("inplace_2buf_gbs") else None,
+            step_frac_of_copy=round(sb / (r["ms"] * 1e-3) / 1e9 / ceiling["copy_nt_gbs"], 4)
+            if ceiling.get("copy_nt_gbs") else None)
+    # MFMA work of the DMM . r step (precision form): 2 (3N)^2 flops per lattice cell
+    if head != "fp64":
+        counts, _ = workloads.cell_counts(spec, med)
+        for m, mat in enumerate(spec["materials"]):
+            n = mat.get("natom", 0)
+            if n and counts[m]:
+                name = "ion_update[m%d,N%d]" % (m, n)
+                kt = [k for k in r["prof"] if k["name"] == name]
+                if kt:
+                    fl = 2.0 * (3 * n) ** 2 * counts[m]
+                    tf = fl / (kt[0]["ms"] * 1e-3) / 1e12
+                    out["mfma"] = {"kernel": name, "op": "v_mfma_f32_16x16x4f32, -DMM . r", "flops": fl,
+                                   "achieved": round(tf, 2), "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": round(tf / MFMA_FP32_PEAK_TFLOPS, 4)}
     if "fp64" in res and head != "fp64":
         r64 = res["fp64"]
         out["fp64"] = {"value": round(cells / (r64["ms"] * 1e-3) / 1e9, 4), "ms_per_step": round(r64["ms"], 4),
