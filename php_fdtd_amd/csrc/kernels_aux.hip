@@ -21,9 +21,11 @@ __device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool res
             for (int q = 0; q < 3; ++q) row[1 + q] = val(q);
             double rv[2] = {0.0, 0.0};
             if (a.mon_ridx >= 0) {
-                if (a.mon_rl) {  // compensated FP32 state
-                    rv[0] = pk_unpack(static_cast<const float *>(a.mon_r)[a.mon_ridx], a.mon_rl[a.mon_ridx]);
-                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx], a.mon_vl[a.mon_vidx]);
+                if (a.mon_rc >= 0) {  // compensated FP32 state
+                    rv[0] = pk_unpack(static_cast<const float *>(a.mon_r)[a.mon_ridx],
+                                      static_cast<const int8_t *>(a.mon_r)[a.mon_rc]);
+                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx],
+                                      static_cast<const int8_t *>(a.mon_v)[a.mon_vc]);
                 } else {
                     rv[0] = static_cast<const double *>(a.mon_r)[a.mon_ridx];
                     rv[1] = static_cast<const double *>(a.mon_v)[a.mon_vidx];
@@ -91,11 +93,13 @@ __global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, 
 }
 
 // max |r| over one lattice material's state (FP64 values, or the FP32 words
-// of compensated state: max |r| rounded to FP32).
+// of compensated state: max |r| rounded to FP32; its compensation words,
+// the last 64 of every 320, are skipped).
 template <typename T>
 __global__ __launch_bounds__(256) void reduce_r(const T *r, int64_t n, unsigned long long *out) {
     unsigned long long m = 0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        if (sizeof(T) == 4 && q % 320 >= 256) continue;
         const unsigned long long b = __double_as_longlong(fabs((double)r[q]));
         m = m > b ? m : b;
     }
@@ -114,10 +118,10 @@ __global__ __launch_bounds__(256) void argmax_r(const T *r, const uint32_t *cell
     unsigned long long best = ~0ull;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (int64_t)gridDim.x * blockDim.x) {
-        if ((unsigned long long)__double_as_longlong(fabs((double)r[q])) != target) continue;
         int64_t s;
         int dof;
-        if (!lay.r_decode(q, nslot, &s, &dof)) continue;
+        if (!lay.r_decode(q, nslot, &s, &dof)) continue;  // padding, compensation words
+        if ((unsigned long long)__double_as_longlong(fabs((double)r[q])) != target) continue;
         const int64_t c = cells[s];
         const int64_t i = c % nxp, j = (c / nxp) % ny, k = c / ((int64_t)nxp * ny);
         const unsigned long long key = (unsigned long long)(((i * ny + j) * nz + k) * 96 + dof);

@@ -205,19 +205,20 @@ __host__ __device__ constexpr int mfma_wave_lds(int na) {
                                                                          : (3 * na + 15) / 16 * 16 * kRowA;
 }
 
-// Compensated FP32 state: the four FP32 words and four compensation bytes of
-// dof group j of a lane (IonLayout w = 4, group j at base + 256 j).
-__device__ __forceinline__ f32x4 ldquad(const float *base, int j) {
-    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * 256));
+// Compensated FP32 state (IonLayout w = 4): group j of a tile is 320 words,
+// the lanes' FP32 words at word 4 lane, their compensation bytes at word
+// 256 + lane.  `base` is the tile's first word.
+__device__ __forceinline__ f32x4 ldquad(const uint32_t *base, int lane, int j) {
+    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * 320 + 4 * lane));
 }
-__device__ __forceinline__ uint32_t ldbytes(const int8_t *base, int j) {
-    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(base + (int64_t)j * 256));
+__device__ __forceinline__ uint32_t ldbytes(const uint32_t *base, int lane, int j) {
+    return __builtin_nontemporal_load(base + (int64_t)j * 320 + 256 + lane);
 }
-__device__ __forceinline__ void stquad(f32x4 v, float *base, int j) {
-    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * 256));
+__device__ __forceinline__ void stquad(f32x4 v, uint32_t *base, int lane, int j) {
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * 320 + 4 * lane));
 }
-__device__ __forceinline__ void stbytes(uint32_t v, int8_t *base, int j) {
-    __builtin_nontemporal_store(v, reinterpret_cast<uint32_t *>(base + (int64_t)j * 256));
+__device__ __forceinline__ void stbytes(uint32_t v, uint32_t *base, int lane, int j) {
+    __builtin_nontemporal_store(v, base + (int64_t)j * 320 + 256 + lane);
 }
 
 // Byte h of a packed dword as a signed value, and its replacement.
@@ -254,8 +255,9 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     }
     if (tile >= ntile) return;  // wave-uniform
     float *buf = lds_a + wave * WL;
-    // dof group j of this lane's slot at tb + 64 W j (elements; bytes for the compensation array)
+    // FP64 state: dof group j of this lane's slot at tb + 128 j; compensated: tile words at tw
     const int64_t tb = tile * 64 * W * (int64_t)NV + W * lane;
+    const int64_t tw = tile * 320 * (int64_t)NV;
     // r^n: FP64 values (FP64 state), or FP32 words + compensation dwords
     double r[PK ? 1 : W * NVX];
     float rw[PK ? W * NVX : 1];
@@ -264,9 +266,10 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     for (int j = 0; j < NVX; ++j)
         if (j < NV) {
             if constexpr (PK) {
-                const f32x4 h4 = ldquad(static_cast<const float *>(rr_) + tb, j);
+                const uint32_t *rb = static_cast<const uint32_t *>(rr_) + tw;
+                const f32x4 h4 = ldquad(rb, lane, j);
                 for (int h = 0; h < 4; ++h) rw[4 * j + h] = h4[h];
-                rc[j] = ldbytes(a.rl + tb, j);
+                rc[j] = ldbytes(rb, lane, j);
             } else {
                 const f64x2 v2 = ldpair(static_cast<const double *>(rr_) + tb, j);
                 r[2 * j] = v2.x;
@@ -362,9 +365,9 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     for (int j = 0; j < NVX; ++j) {
         if (j >= NV) break;
         if constexpr (PK) {
-            float *rh = static_cast<float *>(rr_) + tb, *vh = static_cast<float *>(vv_) + tb;
-            f32x4 v4 = ldquad(vh, j), r4;
-            uint32_t vc = ldbytes(a.vl + tb, j), rcn = rc[j];
+            uint32_t *rb = static_cast<uint32_t *>(rr_) + tw, *vb = static_cast<uint32_t *>(vv_) + tw;
+            f32x4 v4 = ldquad(vb, lane, j), r4;
+            uint32_t vc = ldbytes(vb, lane, j), rcn = rc[j];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const int q = 4 * j + h;
@@ -384,10 +387,10 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                 v4[h] = hw;
                 vc = with_byte(vc, h, lo);
             }
-            stquad(r4, rh, j);
-            stbytes(rcn, a.rl + tb, j);
-            stquad(v4, vh, j);
-            stbytes(vc, a.vl + tb, j);
+            stquad(r4, rb, lane, j);
+            stbytes(rcn, rb, lane, j);
+            stquad(v4, vb, lane, j);
+            stbytes(vc, vb, lane, j);
         } else {
             double *rd = static_cast<double *>(rr_) + tb, *vd = static_cast<double *>(vv_) + tb;
             f64x2 v2 = ldpair(vd, j);
@@ -502,7 +505,7 @@ template <>
 void launch_ion<float>(const SlotArgs<float> &a, hipStream_t st) {
     const int64_t ntile = (a.nslot + 63) / 64;
     if (!ntile) return;
-    const bool pk = a.rl != nullptr;  // compensated FP32 state
+    const bool pk = a.pk != 0;  // compensated FP32 state
     // the host lays out aop/qop for the instantiation mfma_inst(natom)
     switch (mfma_inst(a.natom)) {
 #define PHPFDTD_MFMA_CASE(NN)                                                                               \
@@ -541,7 +544,7 @@ int64_t slot_blocks(const SlotArgs<real> &a) {
 template <typename real>
 int slot_kind(const SlotArgs<real> &a) {
     if (a.natom == 0) return 0;
-    if (sizeof(real) == 4) return mfma_inst(a.natom) + (a.rl ? kPackedKind : 0);
+    if (sizeof(real) == 4) return mfma_inst(a.natom) + (a.pk ? kPackedKind : 0);
     switch (a.natom) {
         case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: return a.natom;
         default: return kMaxAtom;

@@ -77,8 +77,7 @@ struct Special {
     std::vector<double> qezv, qezm, dmm;
     uint32_t *d_cells = nullptr;
     void *d_P = nullptr, *d_Po = nullptr;
-    void *d_r = nullptr, *d_v = nullptr;        // FP64 values, or FP32 words of compensated state
-    int8_t *d_rl = nullptr, *d_vl = nullptr;    // compensation bytes (compensated state only)
+    void *d_r = nullptr, *d_v = nullptr;        // lattice state, layout `lay`
     double *d_dmm = nullptr, *d_qezv = nullptr, *d_qezm = nullptr;
     float *d_aop = nullptr, *d_qop = nullptr;  // MFMA operand tables (precision form)
     SlotArgs<double> sa64{};                    // argument block of its slot kernel (current plan)
@@ -144,7 +143,7 @@ struct phpfdtd_ctx {
     bool mon_on = false;
     int64_t mon_c = -1, mon_ridx = -1, mon_vidx = -1;
     const void *mon_r = nullptr, *mon_v = nullptr;
-    const int8_t *mon_rl = nullptr, *mon_vl = nullptr;
+    int64_t mon_rc = -1, mon_vc = -1;
     unsigned long long *d_diag = nullptr;
     float *d_slice = nullptr;
     // near-field DFT of one plane
@@ -248,7 +247,7 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
         if (ade[m]) l.slots += 2 * 3 * n * rs;
         if (natom[m]) {
             const IonLayout il = ion_layout(natom[m], pk);
-            l.slots += (n + 63) / 64 * (size_t)(il.tile_r() + il.tile_v()) * (pk ? 5 : 8) + 9 * natom[m] * natom[m] * 8 +
+            l.slots += (n + 63) / 64 * (size_t)(il.tile_r() + il.tile_v()) * (pk ? 4 : 8) + 9 * natom[m] * natom[m] * 8 +
                        18 * natom[m] * 8;
             if (mfma)
                 l.slots += ((size_t)mfma_nma(natom[m]) * mfma_nk(natom[m]) * 64 + 18 * natom[m]) * sizeof(float);
@@ -429,7 +428,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         sa.Po = (real *)sp.d_Po;
         sa.natom = sp.pre.natom;
         sa.r = sp.d_r; sa.v = sp.d_v;
-        sa.rl = sp.d_rl; sa.vl = sp.d_vl;
+        sa.pk = x->pk ? 1 : 0;
         sa.lay = sp.lay;
         sa.dmm = sp.d_dmm; sa.qezv = sp.d_qezv; sa.qezm = sp.d_qezm;
         sa.aop = sp.d_aop; sa.qop = sp.d_qop;
@@ -476,8 +475,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
     for (int q = 0; q < 9; ++q) se.F[q] = x->F[q];
     se.mon_r = x->mon_r;
     se.mon_v = x->mon_v;
-    se.mon_rl = x->mon_rl;
-    se.mon_vl = x->mon_vl;
+    se.mon_rc = x->mon_rc;
+    se.mon_vc = x->mon_vc;
     se.mon_ridx = x->mon_ridx;
     se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
@@ -737,13 +736,9 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
         if (sp.pre.natom > 0) {
             sp.lay = ion_layout(sp.pre.natom, x->pk);
             const size_t ntile = (n + 63) / 64;  // 64-slot tiles
-            const size_t es = x->pk ? 4 : 8;
+            const size_t es = x->pk ? 4 : 8;  // bytes per element (IonLayout)
             if ((rc = x->alloc(&sp.d_r, ntile * sp.lay.tile_r() * es))) return rc;
             if ((rc = x->alloc(&sp.d_v, ntile * sp.lay.tile_v() * es))) return rc;
-            if (x->pk) {
-                if ((rc = x->alloc((void **)&sp.d_rl, ntile * sp.lay.tile_r()))) return rc;
-                if ((rc = x->alloc((void **)&sp.d_vl, ntile * sp.lay.tile_v()))) return rc;
-            }
             if ((rc = x->upload((void **)&sp.d_dmm, sp.dmm))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezv, sp.qezv))) return rc;
             if ((rc = x->upload((void **)&sp.d_qezm, sp.qezm))) return rc;
@@ -835,7 +830,7 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
     x->mon_c = zf < x->nz ? (int64_t)x->fidx(mx, my, zf) : -1;
     x->mon_ridx = x->mon_vidx = -1;
     x->mon_r = x->mon_v = nullptr;
-    x->mon_rl = x->mon_vl = nullptr;
+    x->mon_rc = x->mon_vc = -1;
     const int m = x->medium[x->fidx(mx, my, mz)];
     const int sp = x->special_of[m];
     if (sp >= 0 && atom < x->pre[m].natom) {
@@ -845,8 +840,10 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
         x->mon_vidx = x->specials[sp].lay.v_index(slot, atom * 3 + dir);
         x->mon_r = x->specials[sp].d_r;
         x->mon_v = x->specials[sp].d_v;
-        x->mon_rl = x->specials[sp].d_rl;
-        x->mon_vl = x->specials[sp].d_vl;
+        if (x->pk) {
+            x->mon_rc = x->specials[sp].lay.c_index(slot, atom * 3 + dir);
+            x->mon_vc = x->mon_rc;
+        }
     }
     touch(x);
     return 0;
@@ -1118,7 +1115,7 @@ int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double 
                 float w;
                 int8_t b;
                 HIPCHK(hipMemcpy(&w, dev + at * 4, 4, hipMemcpyDeviceToHost));
-                HIPCHK(hipMemcpy(&b, (k == 0 ? s.d_rl : s.d_vl) + at, 1, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&b, dev + s.lay.c_index(slot, q), 1, hipMemcpyDeviceToHost));
                 out[q] = pk_unpack(w, b);
             } else {
                 HIPCHK(hipMemcpy(out + q, dev + at * 8, 8, hipMemcpyDeviceToHost));
@@ -1191,13 +1188,13 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     const size_t n = s.cells.size();
     int width;
     void *dev;
-    int8_t *devl = nullptr;  // compensation bytes of compensated lattice state
+    bool pk = false;  // compensated lattice state: FP32 words and compensation bytes in one array
     size_t es;
     if (kind < 2) {
         if (!s.pre.natom) return fail(PHPFDTD_EINVAL, "material %d has no lattice", mat);
         width = 3 * s.pre.natom;
         dev = kind == 0 ? s.d_r : s.d_v;
-        devl = kind == 0 ? s.d_rl : s.d_vl;
+        pk = x->pk;
         es = x->pk ? 4 : 8;
     } else {
         if (!s.pre.ade) return fail(PHPFDTD_EINVAL, "material %d has no ADE state", mat);
@@ -1209,27 +1206,24 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     const bool tiled = kind < 2;  // lattice r, v in 64-slot tiles; ADE P, Po as [3][nslot]
     const size_t nelem = tiled ? (n + 63) / 64 * (size_t)(kind == 0 ? s.lay.tile_r() : s.lay.tile_v()) : n * width;
     std::vector<unsigned char> h(nelem * es);
-    std::vector<int8_t> hl(devl ? nelem : 0);
     const std::vector<int64_t> ord = file_order(x, s);
-    if (!write || tiled) {
-        HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
-        if (devl) HIPCHK(hipMemcpy(hl.data(), devl, hl.size(), hipMemcpyDeviceToHost));
-    }
+    if (!write || tiled) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < n; ++q) {
         const int64_t slot = ord[q];
         for (int d = 0; d < width; ++d) {
             const size_t hi = tiled ? (size_t)(kind == 0 ? s.lay.r_index(slot, d) : s.lay.v_index(slot, d))
                                     : (size_t)d * n + slot;
             double &u = buf[q * width + d];
-            if (devl) {  // compensated FP32: written values are rounded to 32 significant bits
+            if (pk) {  // compensated FP32: written values are rounded to 32 significant bits
+                const size_t ci = (size_t)s.lay.c_index(slot, d);
                 float w;
                 int b;
                 if (write) {
                     pk_pack(u, &w, &b);
                     ((float *)h.data())[hi] = w;
-                    hl[hi] = (int8_t)b;
+                    ((int8_t *)h.data())[ci] = (int8_t)b;
                 } else {
-                    u = pk_unpack(((float *)h.data())[hi], hl[hi]);
+                    u = pk_unpack(((float *)h.data())[hi], ((int8_t *)h.data())[ci]);
                 }
             } else if (es == 8) {
                 if (write) ((double *)h.data())[hi] = u;
@@ -1240,10 +1234,7 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
             }
         }
     }
-    if (write) {
-        HIPCHK(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
-        if (devl) HIPCHK(hipMemcpy(devl, hl.data(), hl.size(), hipMemcpyHostToDevice));
-    }
+    if (write) HIPCHK(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -95,29 +95,39 @@ inline int mfma_inst(int natom) {
 }
 
 // Lattice state layout of one material: tiles of 64 slots, the 3N dofs of a
-// slot taken in groups of w, r[tile][dof / w][64][w].  A lane owns a slot and
-// reads a group as one 16-byte vector, so every wavefront access is one 1 KiB
-// row:
-//   * FP64 state (plain form, and the precision form with FP64 r, v): w = 2;
-//   * compensated FP32 state (precision form): w = 4, FP32 words in one array
-//     and their int8 compensation bytes in a second array with the same
-//     indices (a lane's group of four bytes is one dword).
-// 3N is padded to a multiple of w with zero dofs.
+// slot taken in groups of w, one 16-byte vector per lane and group, so every
+// wavefront access is one 1 KiB row.  3N is padded to a multiple of w with
+// zero dofs.
+//   * FP64 state (plain form, and the precision form with FP64 r, v): w = 2,
+//     r[tile][dof / 2][64][2] doubles.
+//   * compensated FP32 state (precision form): w = 4; group j of a tile is
+//     1280 bytes: the 64 lanes' four FP32 words (1 KiB), then the 64 lanes'
+//     four compensation bytes (one dword per lane, 256 B).  Words and bytes of
+//     an array stream from one address range.
 struct IonLayout {
     int n3;
     int w;  // dofs per lane vector: 2 (FP64) or 4 (compensated FP32)
     __host__ __device__ int n3p() const { return (n3 + w - 1) / w * w; }
-    __host__ __device__ int64_t tile_r() const { return 64LL * n3p(); }
+    // elements per tile: doubles (FP64), or 4-byte words (compensated: 80 per
+    // padded dof, 64 FP32 words and 16 words of compensation bytes)
+    __host__ __device__ int64_t tile_r() const { return (w == 4 ? 80LL : 64LL) * n3p(); }
     __host__ __device__ int64_t tile_v() const { return tile_r(); }
+    // element of dof q of slot s: the double, or the FP32 word
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
-        return (s >> 6) * tile_r() + (int64_t)(q / w) * 64 * w + (s & 63) * w + q % w;
+        return (s >> 6) * tile_r() + (int64_t)(q / w) * (w == 4 ? 320 : 128) + (s & 63) * w + q % w;
     }
     __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
-    // slot and dof of r element idx; false for padding
+    // byte offset of the compensation byte of dof q of slot s (compensated state)
+    __host__ __device__ int64_t c_index(int64_t s, int q) const {
+        return ((s >> 6) * tile_r() + (int64_t)(q / 4) * 320 + 256 + (s & 63)) * 4 + q % 4;
+    }
+    // slot and dof of element idx; false for padding and compensation words
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
         const int64_t t = idx / tile_r(), rem = idx % tile_r();
-        const int u = (int)(rem % (64 * w));
-        *q = (int)(rem / (64 * w)) * w + u % w;
+        const int gsz = w == 4 ? 320 : 128;
+        const int u = (int)(rem % gsz);
+        if (u >= 64 * w) return false;
+        *q = (int)(rem / gsz) * w + u % w;
         *s = t * 64 + u / w;
         return *q < n3 && *s < nslot;
     }
@@ -179,8 +189,8 @@ struct SlotArgs {
     real *P, *Po;  // [3][nslot]
     // lattice (fdtd.c:3505-3528)
     int natom;
-    void *r, *v;             // FP64 state (double), or the FP32 words of compensated state
-    int8_t *rl, *vl;         // compensation bytes of compensated state, else null
+    void *r, *v;             // lattice state, layout `lay`
+    int pk;                  // 1: compensated FP32 state (lay.w == 4), 0: FP64
     IonLayout lay;
     const double *qezv;      // [N][3][3]
     const double *qezm;      // [N][3][3]
@@ -201,9 +211,9 @@ struct StepEndArgs {
     int mon_on;
     int64_t mon_c;     // field index sampled (z+1 in reference mode), -1 = zero halo
     const void *F[9];  // field arrays (real)
-    const void *mon_r, *mon_v;    // lattice state of the monitor cell's material (double, or
-    const int8_t *mon_rl, *mon_vl;  // float words + compensation bytes when these are set)
-    int64_t mon_ridx, mon_vidx;   // offsets of (slot, atom*3+dir) in r and v, -1 = no R/V
+    const void *mon_r, *mon_v;    // lattice state of the monitor cell's material
+    int64_t mon_ridx, mon_vidx;   // elements of (slot, atom*3+dir) in r and v, -1 = no R/V
+    int64_t mon_rc, mon_vc;       // their compensation bytes (compensated state), else -1
     double dt;
     int part;                     // 0 whole row, 1 D and H, 2 t, E, R, V + step advance
 };
