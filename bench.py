@@ -18,6 +18,7 @@ import argparse
 import json
 import math
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -61,8 +62,13 @@ def profile_traffic(symbol):
     or None when the summary has no row for it."""
     if not os.path.exists(PROFILE):
         return None
+    text = open(PROFILE).read()
+    m = re.search(r"over the (\d+) steps of each form", text)
+    if not m:
+        return None
+    steps = int(m.group(1))  # steps each form ran in the counter runs
     in_pmc = False
-    for ln in open(PROFILE):
+    for ln in text.splitlines():
         if ln.startswith("## "):
             in_pmc = ln.startswith("## HBM traffic")
         cells = [c.strip() for c in ln.strip().strip("|").split("|")]
@@ -71,7 +77,6 @@ def profile_traffic(symbol):
                 launches, traffic = int(cells[1]), float(cells[4])
             except ValueError:
                 return None
-            steps = 6  # the PMC runs step each form 1 + 2 + 3 times
             per_step = max(1, launches // steps)
             return traffic * 1e9 / per_step
     return None
