@@ -137,7 +137,7 @@ def central(a):
     return a[(i > n0 * 0.25) & (i < n0 * 0.75) & (j > n1 * 0.25) & (j < n1 * 0.75)]
 
 
-def accuracy(name, interval=10):
+def accuracy(name, interval=10, steps=None):
     """FP32 precision form vs FP64 plain form over a full-length run: monitor
     trace relative L2 (fields; lattice R, V) and the near-field single-bin DFT
     at the source frequency of the z plane one cell above the slab, central
@@ -146,6 +146,8 @@ def accuracy(name, interval=10):
     from php_fdtd_amd.sim import Simulation
 
     spec, med = workloads.CONFIGS[name]()
+    if steps:
+        spec = workloads.with_steps(spec, steps)
     g = spec["grid"]
     T = g["tsteps"]
     N = T // interval
@@ -181,8 +183,8 @@ def accuracy(name, interval=10):
         return float(np.linalg.norm(u - v) / n) if n else None
 
     return {
-        "workload": "%s full length: %dx%dx%d cells, %d steps, FP32 form vs FP64 plain form" % (
-            name, g["nx"], g["ny"], g["nz"], T),
+        "workload": "%s%s: %dx%dx%d cells, %d steps, FP32 form vs FP64 plain form" % (
+            name, "" if steps else " full length", g["nx"], g["ny"], g["nz"], T),
         "monitor_rel_l2": rl2(a["rows"][:, 1:10], b["rows"][:, 1:10]),
         "monitor_rv_rel_l2": rl2(a["rows"][:, 10:12], b["rows"][:, 10:12]),
         "rv_fp64_storage": {"monitor_rel_l2": rl2(c["rows"][:, 1:10], b["rows"][:, 1:10]),
@@ -315,6 +317,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64", help="comma list of fp32, fp32rv64, fp64")
     ap.add_argument("--accuracy", default="cfg1", help="config for the FP32-vs-FP64 accuracy run ('' = skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy run (0 = the config's own)")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving mix (0 = skip)")
     ap.add_argument("--serving-step-scale", type=float, default=0.05)
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false",
@@ -425,7 +428,7 @@ def main():
             den = np.linalg.norm(b[:n, 1:10])
             out["fp32_monitor_rel_l2_vs_fp64"] = float(num / den) if den else None
     if args.accuracy:
-        out["accuracy"] = accuracy(args.accuracy)
+        out["accuracy"] = accuracy(args.accuracy, steps=args.accuracy_steps or None)
     if args.serving:
         out["serving"] = serving(args.serving, args.serving_step_scale)
     if not args.no_cpu:
