@@ -11,17 +11,18 @@
 //   a = QeZM^T E_cr - DMM r^n;  r += v dt + a dt^2/2;  v += a dt  (3505-3528)
 //   E = R^T E_cr                                                  (3489-3503)
 //
-// Lattice state layout (IonLayout), both forms: tiles of 64 slots with the
-// dofs in pairs, r[tile][dof / 2][64][2], so a lane owns a slot, reads two
-// dofs as one 16-byte vector and every wavefront access is 1 KiB contiguous,
-// streamed non-temporally.
+// Lattice state layout (IonLayout): tiles of 64 slots, a lane owns a slot and
+// moves a group of its dofs as one 16-byte vector, so every wavefront access
+// is 1 KiB contiguous, streamed non-temporally: FP64 values in dof pairs, or
+// compensated FP32 (an FP32 word and an 8-bit correction per value) in dof
+// quads with the correction bytes after each 1 KiB group.
 //
 // Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
 // DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
 // Precision form (FP32): the same slot-per-lane streams; -DMM r^n runs on FP32
 // MFMA (v_mfma_f32_16x16x4f32) after an LDS transpose of r^n rounded to FP32,
 // accumulated in FP32; P_ion and QeZM^T E_cr on the VALU in FP32.  The r, v
-// integration stays FP64.
+// integration stays FP64; r, v are stored compensated (PK) or in FP64.
 #include "phpfdtd_internal.h"
 
 namespace phpfdtd {
@@ -172,7 +173,7 @@ __device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const do
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Precision form: lane l of a wavefront owns slot 64 tile + l and all 3N dofs
-// of its r^n and v^n - the memory pattern of the plain form (dof pairs as
+// of its r^n and v^n - the memory pattern of the plain form (dof groups as
 // 16-byte vectors, every access one 1 KiB row, all of r^n requested up front).  Only the
 // contraction -DMM . r^n runs on FP32 MFMA (v_mfma_f32_16x16x4f32; lane l on
 // gfx950: A[m][k] in lane m + 16k, B[k][n] in lane n + 16k, D[m][n] in lane
