@@ -89,3 +89,43 @@ def test_reference_input_errors(lib, mutate, msg):
     rc = lib.phpfdtd_create(ctypes.byref(grid), mats.arr, mats.n, _abi.i32p(_abi.medium_array(med)), ctypes.byref(h))
     assert rc == _abi.EINVAL
     assert msg in lib.phpfdtd_last_error()
+
+
+def test_footprint_by_precision(lib):
+    """FP64 > FP32 with FP64 r, v > FP32 with compensated r, v; the last two
+    differ by the lattice state alone: 64-slot tiles of 3N (here 12) dofs,
+    r and v, 8 bytes a value against 5 (80 words of 4 bytes per 64 values)."""
+    spec, med = _case()
+    mats = _abi.MaterialTable(spec["materials"])
+    m = _abi.medium_array(med)
+    fp = {}
+    for p in (_abi.FP64, _abi.FP32_RV64, _abi.FP32):
+        grid = _abi.make_grid(spec["grid"], p)
+        out = ctypes.c_size_t()
+        assert lib.phpfdtd_footprint_bytes(ctypes.byref(grid), mats.arr, mats.n, _abi.i32p(m), ctypes.byref(out)) == 0
+        fp[p] = out.value
+    assert fp[_abi.FP64] > fp[_abi.FP32_RV64] > fp[_abi.FP32]
+    lat = [i for i, mat in enumerate(spec["materials"]) if mat["natom"]]
+    assert len(lat) == 1
+    n3 = 3 * spec["materials"][lat[0]]["natom"]
+    ntile = (int((med == lat[0]).sum()) + 63) // 64
+    n3_pairs, n3_quads = (n3 + 1) // 2 * 2, (n3 + 3) // 4 * 4
+    assert fp[_abi.FP32_RV64] - fp[_abi.FP32] == ntile * 2 * (64 * n3_pairs * 8 - 80 * n3_quads * 4)
+
+
+def test_rejects_bad_precision_and_atoms(lib):
+    spec, med = _case()
+    mats = _abi.MaterialTable(spec["materials"])
+    grid = _abi.make_grid(spec["grid"], 3)
+    h = ctypes.c_void_p()
+    rc = lib.phpfdtd_create(ctypes.byref(grid), mats.arr, mats.n, _abi.i32p(_abi.medium_array(med)), ctypes.byref(h))
+    assert rc == _abi.EINVAL and b"precision" in lib.phpfdtd_last_error()
+    b = ctypes.c_void_p()
+    assert lib.phpfdtd_batch_create(3, ctypes.byref(b)) == _abi.EINVAL
+    mats_spec = [dict(mt) for mt in spec["materials"]]
+    lat = [i for i, mt in enumerate(mats_spec) if mt["natom"]][0]
+    mats_spec[lat]["natom"] = 33
+    mats = _abi.MaterialTable(mats_spec)
+    grid = _abi.make_grid(spec["grid"], _abi.FP32)
+    rc = lib.phpfdtd_create(ctypes.byref(grid), mats.arr, mats.n, _abi.i32p(_abi.medium_array(med)), ctypes.byref(h))
+    assert rc == _abi.EINVAL
