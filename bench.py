@@ -232,12 +232,14 @@ def serving(n, step_scale, seed=4, load=0.8):
     poi = serve.Server(_abi.FP32).serve(requests(list(arr)))
     st_poi = serve.summarize(poi)
     log("  serving: FP32 batched, Poisson %.2f/s: %s" % (rate, st_poi))
-    errs = []
+    errs, ids = [], []
     by_id = {d["id"]: rows for d, rows in zip(descs, seq_rows)}
     for r in sat:
         ref = by_id[r.id][:, 1:10]
         den = np.linalg.norm(ref)
         errs.append(float(np.linalg.norm(r.rows[:, 1:10] - ref) / den) if den else 0.0)
+        ids.append(r.id)
+    worst = {d["id"]: d for d in descs}[ids[int(np.argmax(errs))]]
     seq_sps = n / seq_t
     return {
         "workload": "serving mix, %d requests (seed %d), steps U(5000, 20000) x %g, FP32 form batched vs FP64 plain "
@@ -249,7 +251,9 @@ def serving(n, step_scale, seed=4, load=0.8):
                                        max_members=int(max(members))),
         "fp32_batched_poisson": dict(st_poi, offered_per_s=round(rate, 4), load=load),
         "solves_ratio_vs_fp64_one_at_a_time": round(st_sat["solves_per_s"] / seq_sps, 3),
-        "per_request_monitor_rel_l2_vs_fp64": {"median": float(np.median(errs)), "max": float(np.max(errs))},
+        "per_request_monitor_rel_l2_vs_fp64": {
+            "median": float(np.median(errs)), "max": float(np.max(errs)),
+            "max_request": {k: worst[k] for k in ("id", "kind", "shape", "wn", "steps")}},
     }
 
 
