@@ -248,18 +248,27 @@ def test_compensated_vs_fp64_state(name):
     assert abs(a[2].max_r - b[2].max_r) <= 1e-5 * max(b[2].max_r, 1e-300)
 
 
-def test_graph_matches_stream_launches(monkeypatch):
-    c = golden_cases.load("hbn")
-    spec = c["spec"]
-    res = []
-    for ng in ("0", "1"):
-        monkeypatch.setenv("PHPFDTD_NO_GRAPH", ng)
-        sim = Simulation(spec, c["medium"], _abi.FP64)
-        sim.run(37)  # 2 x 16-step graph + 5 x 1-step graph
-        res.append([sim.field(q) for q in range(9)])
-        sim.close()
-    for q in range(9):
-        np.testing.assert_array_equal(res[0][q], res[1][q])
+@pytest.mark.parametrize("case", ["hbn", "cfg3/8"])
+def test_graph_matches_stream_launches(monkeypatch, case):
+    """16-step graphs (with the next step's lattice-free H ranges on a second
+    stream beside the lattice kernel) give the same bits as plain sequential
+    launches, with and without an ADE material beside the lattice."""
+    if case == "hbn":
+        c = golden_cases.load("hbn")
+        spec, med, precs = c["spec"], c["medium"], (_abi.FP64,)
+    else:  # 1024x1024x640 layout at 1/8 scale: lattice slab + Drude ADE cone
+        spec, med = workloads.config3(8)
+        spec, precs = workloads.with_steps(spec, 40), (_abi.FP64, _abi.FP32)
+    for prec in precs:
+        res = []
+        for ng in ("0", "1"):
+            monkeypatch.setenv("PHPFDTD_NO_GRAPH", ng)
+            sim = Simulation(spec, med, prec)
+            sim.run(37)  # 2 x 16-step graph + 5 x 1-step graph
+            res.append([sim.field(q) for q in range(9)])
+            sim.close()
+        for q in range(9):
+            np.testing.assert_array_equal(res[0][q], res[1][q])
 
 
 def test_errors_and_bounds():
