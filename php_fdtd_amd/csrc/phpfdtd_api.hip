@@ -630,8 +630,11 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     x->use_graph = !(ng && ng[0] == '1');
     const char *no = getenv("PHPFDTD_NO_OVERLAP");
     x->allow_overlap = !(no && no[0] == '1');
-    // Yee tile shape (defaults measured on MI355X, 1024x1024x640); overridable for tuning
-    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 8 : 16;
+    // Yee tile shape, defaults measured on MI355X (1024x1024x640, same-context
+    // comparisons): FP32 1 y row x 4 planes, FP64 2 y rows x 16 planes;
+    // overridable for tuning
+    x->yee_rows = x->prec == PHPFDTD_FP32 ? 1 : 2;
+    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 4 : 16;
     if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
         const int v = atoi(e);
         if (v == 1 || v == 2 || v == 4) x->yee_rows = v;
@@ -1537,8 +1540,11 @@ int phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (x->g.precision != b->form)
         return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->g.precision, b->form);
     if (x->device != b->device) return fail(PHPFDTD_EINVAL, "context is on device %d, batch on %d", x->device, b->device);
-    if (x->yee_rows != 1) return fail(PHPFDTD_EINVAL, "batched launches use one y row per wavefront");
     HIPCHK(hipSetDevice(x->device));
+    if (x->yee_rows != 1) {  // batched Yee launches take one y row per wavefront
+        x->yee_rows = 1;
+        x->dirty = true;
+    }
     if (!x->ev) HIPCHK(hipEventCreateWithFlags(&x->ev, hipEventDisableTiming));
     b->members.push_back(x);
     x->batch = b;
