@@ -227,7 +227,9 @@ int phpfdtd_read_dft(phpfdtd_ctx *ctx, double *re, double *im, int *n_samples);
  * work); phpfdtd_run / phpfdtd_profile / phpfdtd_time_steps on a member
  * return PHPFDTD_ESTATE.  Membership may change between phpfdtd_batch_run
  * calls; destroying a member removes it.  Members must share the batch's
- * precision.  Not thread-safe together with calls on its members. */
+ * precision value.  A member's Yee launches take one y row per wavefront
+ * from phpfdtd_batch_add on (same results).  Not thread-safe together with
+ * calls on its members. */
 typedef struct phpfdtd_batch phpfdtd_batch;
 int  phpfdtd_batch_create(int precision, phpfdtd_batch **out);
 int  phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *ctx);
@@ -250,6 +252,9 @@ int phpfdtd_write_field(phpfdtd_ctx *ctx, int comp, const double *in);
 /* Whole state of the ion lattice / ADE polarisation in file order:
  * kind 0 = r, 1 = v (3*natom doubles per cell of material mat),
  *      2 = P_ADE, 3 = P_ADE old (3 doubles per cell of material mat).
+ * In PHPFDTD_FP32, r and v are stored compensated: written values are
+ * rounded to 32 significant bits (relative error <= 2^-31), and reading
+ * returns the stored values exactly.
  * Cells of other materials are skipped: n_cells*width values are exchanged,
  * where n_cells is returned by phpfdtd_count_cells. */
 int phpfdtd_count_cells(phpfdtd_ctx *ctx, int mat, int64_t *n_cells);

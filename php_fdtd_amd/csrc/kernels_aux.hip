@@ -94,12 +94,12 @@ __global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, 
 
 // max |r| over one lattice material's state (FP64 values, or the FP32 words
 // of compensated state: max |r| rounded to FP32; its compensation words,
-// the last 64 of every 320, are skipped).
+// the last kPkGroup - kPkWords of every kPkGroup, are skipped).
 template <typename T>
 __global__ __launch_bounds__(256) void reduce_r(const T *r, int64_t n, unsigned long long *out) {
     unsigned long long m = 0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        if (sizeof(T) == 4 && q % 320 >= 256) continue;
+        if (sizeof(T) == 4 && q % kPkGroup >= kPkWords) continue;
         const unsigned long long b = __double_as_longlong(fabs((double)r[q]));
         m = m > b ? m : b;
     }

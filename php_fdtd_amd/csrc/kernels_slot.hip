@@ -206,20 +206,20 @@ __host__ __device__ constexpr int mfma_wave_lds(int na) {
                                                                          : (3 * na + 15) / 16 * 16 * kRowA;
 }
 
-// Compensated FP32 state (IonLayout w = 4): group j of a tile is 320 words,
-// the lanes' FP32 words at word 4 lane, their compensation bytes at word
-// 256 + lane.  `base` is the tile's first word.
+// Compensated FP32 state (IonLayout w = 4): group j of a tile is kPkGroup
+// words, the lanes' FP32 words at word 4 lane, their compensation bytes at
+// word kPkWords + lane.  `base` is the tile's first word.
 __device__ __forceinline__ f32x4 ldquad(const uint32_t *base, int lane, int j) {
-    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * 320 + 4 * lane));
+    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * kPkGroup + 4 * lane));
 }
 __device__ __forceinline__ uint32_t ldbytes(const uint32_t *base, int lane, int j) {
-    return __builtin_nontemporal_load(base + (int64_t)j * 320 + 256 + lane);
+    return __builtin_nontemporal_load(base + (int64_t)j * kPkGroup + kPkWords + lane);
 }
 __device__ __forceinline__ void stquad(f32x4 v, uint32_t *base, int lane, int j) {
-    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * 320 + 4 * lane));
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * kPkGroup + 4 * lane));
 }
 __device__ __forceinline__ void stbytes(uint32_t v, uint32_t *base, int lane, int j) {
-    __builtin_nontemporal_store(v, base + (int64_t)j * 320 + 256 + lane);
+    __builtin_nontemporal_store(v, base + (int64_t)j * kPkGroup + kPkWords + lane);
 }
 
 // Byte h of a packed dword as a signed value, and its replacement.
@@ -258,7 +258,7 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     float *buf = lds_a + wave * WL;
     // FP64 state: dof group j of this lane's slot at tb + 128 j; compensated: tile words at tw
     const int64_t tb = tile * 64 * W * (int64_t)NV + W * lane;
-    const int64_t tw = tile * 320 * (int64_t)NV;
+    const int64_t tw = tile * kPkGroup * (int64_t)NV;
     // r^n: FP64 values (FP64 state), or FP32 words + compensation dwords
     double r[PK ? 1 : W * NVX];
     float rw[PK ? W * NVX : 1];
