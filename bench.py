@@ -216,12 +216,16 @@ def serving(n, step_scale, seed=4, load=0.8):
 
     work = float(sum(med.size * sp["grid"]["tsteps"] for sp, med in cases))
     seq_rows, seq_t = [], 0.0
-    for r in requests([0.0] * n):
+    last = time.perf_counter()
+    for q, r in enumerate(requests([0.0] * n)):
         rows, t = serve.run_alone(r, _abi.FP64)
         seq_rows.append(rows)
         seq_t += t
+        if time.perf_counter() - last >= 30:
+            last = time.perf_counter()
+            log("  serving: FP64 one at a time, %d of %d done, %.0f s" % (q + 1, n, seq_t))
     log("  serving: FP64 one at a time %.2f s" % seq_t)
-    srv = serve.Server(_abi.FP32)
+    srv = serve.Server(_abi.FP32, progress=30)
     sat = srv.serve(requests([0.0] * n))
     st_sat = serve.summarize(sat)
     members = [m for _, m, _ in srv.trace]
@@ -229,7 +233,7 @@ def serving(n, step_scale, seed=4, load=0.8):
     rate = load * st_sat["solves_per_s"]
     arr = np.cumsum([d["gap"] for d in descs]) / rate
     arr -= arr[0]
-    poi = serve.Server(_abi.FP32).serve(requests(list(arr)))
+    poi = serve.Server(_abi.FP32, progress=30).serve(requests(list(arr)))
     st_poi = serve.summarize(poi)
     log("  serving: FP32 batched, Poisson %.2f/s: %s" % (rate, st_poi))
     errs, ids = [], []

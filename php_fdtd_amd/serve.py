@@ -25,6 +25,7 @@ same per-tile code on the member's own arguments.
 """
 import collections
 import concurrent.futures as cf
+import sys
 import threading
 import time
 
@@ -67,8 +68,9 @@ def _build(req, precision):
 
 class Server:
     def __init__(self, precision=_abi.FP32, hbm_budget=0.85 * HBM_BYTES, max_members=64, max_chunk=256,
-                 workers=4):
+                 workers=4, progress=None):
         self.precision = precision
+        self.progress = progress  # seconds between status lines on stderr (None: quiet)
         self.budget = float(hbm_budget)
         self.max_members = int(max_members)
         self.max_chunk = int(max_chunk)
@@ -103,6 +105,8 @@ class Server:
         finishing = []
         nxt = 0
         t0 = time.perf_counter()
+        last = t0
+        ndone = 0
         try:
             while nxt < len(reqs) or pending or building or active:
                 now = time.perf_counter()
@@ -162,6 +166,12 @@ class Server:
                 for a in active:
                     a[2] -= chunk
                 done = [a for a in active if a[2] == 0]
+                ndone += len(done)
+                if self.progress and time.perf_counter() - last >= self.progress:
+                    last = time.perf_counter()
+                    print("serve: %.0f s, %d done, %d running, %d building, %d queued" % (
+                        last - t0, ndone, len(active) - len(done), len(building), len(pending) + len(reqs) - nxt),
+                          file=sys.stderr, flush=True)
                 if done:
                     active = [a for a in active if a[2] > 0]
                     for r, sim, _ in done:
