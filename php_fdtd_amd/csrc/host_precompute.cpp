@@ -185,11 +185,14 @@ int material_coefs(const phpfdtd_grid &g, const phpfdtd_material &m, MatPre *o, 
             gp[e] = gam[e] * g.dt / 2.0 + ((e % 4) == 0 ? 1.0 : 0.0);
             o->Gm[e] = -gam[e] * g.dt / 2.0 + ((e % 4) == 0 ? 1.0 : 0.0);
             o->W[e] = g.eps0 * g.dt * g.dt * wp2[e];
+            o->Ga[e] = gam[e] * g.dt / 2.0;
         }
         if (inverse3(gp, o->Gpi) != 0) {
             *err = "Error: gdt2p1 can not be inversed";
             return PHPFDTD_EINVAL;
         }
+        // I - Gpi = Gamma (I + Gamma)^-1, the small matrix of the increment form
+        for (int e = 0; e < 9; ++e) o->Gb[e] = ((e % 4) == 0 ? 1.0 : 0.0) - o->Gpi[e];
         const double f = (1.0 - o->lt) / (g.eps0 * kEpsInfHost);
         for (int e = 0; e < 9; ++e) o->A[e] = ((e % 4) == 0) ? f : 0.0;
     } else {

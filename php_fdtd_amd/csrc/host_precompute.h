@@ -26,6 +26,7 @@ struct MatPre {
     double M[9];     // E_lab = M D_lab (plain cells)
     double Lk[9];    // D loss matrix (fdtd.c:3349-3353)
     double Gpi[9], Gm[9], W[9];  // ADE recurrence
+    double Ga[9], Gb[9];         // increment form: Gamma = gamma dt / 2, I - Gpi
     double inv_re[9], inv_im[9];
     double lf, lt;
     bool loss, special, ade;

@@ -62,21 +62,41 @@ __device__ __forceinline__ void stpair(f64x2 v, double *base, int p) {
 }
 
 // ADE recurrence in the crystal frame for one slot; returns P^{n+1}.
+// FP64: P^{n+1} = Gpi (2 P - Gm P^{n-1} + W E) on (P, P^{n-1}), the
+// reference's order (fdtd.c:3463-3479).  FP32: the same recurrence in
+// increment form on (P, J = P - P^{n-1}) - the P^n terms cancel exactly -
+// u = J - Ga J + W E, J' = u - Gb u, P' = P + J': every correction is a small
+// term added to its O(1) value, so the marginal mode of P (the Drude
+// integrator, eigenvalue 1) stays exact in FP32, where the two-level form
+// rounds Gm = I - Ga and Gpi to 24 bits and drifts.
 template <typename real>
 __device__ __forceinline__ void ade_step(const SlotArgs<real> &a, int64_t s, const real Eo[3], real Pn[3]) {
     const int64_t n = a.nslot;
     real P[3], Po[3], tv[3];
     for (int e = 0; e < 3; ++e) {
         P[e] = a.P[e * n + s];
-        Po[e] = a.Po[e * n + s];
+        Po[e] = a.Po[e * n + s];  // P^{n-1} (FP64) or J (FP32)
     }
-    for (int e = 0; e < 3; ++e)
-        tv[e] = 2 * P[e] - (a.Gm[e * 3] * Po[0] + a.Gm[e * 3 + 1] * Po[1] + a.Gm[e * 3 + 2] * Po[2]) +
-                (a.W[e * 3] * Eo[0] + a.W[e * 3 + 1] * Eo[1] + a.W[e * 3 + 2] * Eo[2]);
-    for (int e = 0; e < 3; ++e) Pn[e] = a.Gpi[e * 3] * tv[0] + a.Gpi[e * 3 + 1] * tv[1] + a.Gpi[e * 3 + 2] * tv[2];
-    for (int e = 0; e < 3; ++e) {
-        a.Po[e * n + s] = P[e];
-        a.P[e * n + s] = Pn[e];
+    if constexpr (sizeof(real) == 4) {
+        real u[3], J[3];
+        for (int e = 0; e < 3; ++e)
+            u[e] = Po[e] - (a.Ga[e * 3] * Po[0] + a.Ga[e * 3 + 1] * Po[1] + a.Ga[e * 3 + 2] * Po[2]) +
+                   (a.W[e * 3] * Eo[0] + a.W[e * 3 + 1] * Eo[1] + a.W[e * 3 + 2] * Eo[2]);
+        for (int e = 0; e < 3; ++e) J[e] = u[e] - (a.Gb[e * 3] * u[0] + a.Gb[e * 3 + 1] * u[1] + a.Gb[e * 3 + 2] * u[2]);
+        for (int e = 0; e < 3; ++e) {
+            Pn[e] = P[e] + J[e];
+            a.Po[e * n + s] = J[e];
+            a.P[e * n + s] = Pn[e];
+        }
+    } else {
+        for (int e = 0; e < 3; ++e)
+            tv[e] = 2 * P[e] - (a.Gm[e * 3] * Po[0] + a.Gm[e * 3 + 1] * Po[1] + a.Gm[e * 3 + 2] * Po[2]) +
+                    (a.W[e * 3] * Eo[0] + a.W[e * 3 + 1] * Eo[1] + a.W[e * 3 + 2] * Eo[2]);
+        for (int e = 0; e < 3; ++e) Pn[e] = a.Gpi[e * 3] * tv[0] + a.Gpi[e * 3 + 1] * tv[1] + a.Gpi[e * 3 + 2] * tv[2];
+        for (int e = 0; e < 3; ++e) {
+            a.Po[e * n + s] = P[e];
+            a.P[e * n + s] = Pn[e];
+        }
     }
 }
 

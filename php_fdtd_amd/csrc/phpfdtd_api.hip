@@ -422,6 +422,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
             sa.Gpi[e] = (real)sp.pre.Gpi[e];
             sa.Gm[e] = (real)sp.pre.Gm[e];
             sa.W[e] = (real)sp.pre.W[e];
+            sa.Ga[e] = (real)sp.pre.Ga[e];
+            sa.Gb[e] = (real)sp.pre.Gb[e];
         }
         sa.ade = sp.pre.ade ? 1 : 0;
         sa.P = (real *)sp.d_P;
@@ -1182,12 +1184,44 @@ static std::vector<int64_t> file_order(const phpfdtd_ctx *x, const Special &s) {
     return idx;
 }
 
+// ADE state of the FP32 form: the device holds P and J = P - P^{n-1} (the
+// increment form of the recurrence); the ABI exchanges P and P^{n-1}.  A
+// write of P keeps P^{n-1} (J moves with P), a write of P^{n-1} sets J.
+static int ade_state_f32(phpfdtd_ctx *x, Special &s, int kind, double *buf, int write) {
+    const size_t n = s.cells.size();
+    std::vector<float> P(3 * n), J(3 * n);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(P.data(), s.d_P, P.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(J.data(), s.d_Po, J.size() * 4, hipMemcpyDeviceToHost));
+    const std::vector<int64_t> ord = file_order(x, s);
+    for (size_t q = 0; q < n; ++q)
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = (size_t)d * n + ord[q];
+            double &u = buf[q * 3 + d];
+            if (!write) {
+                u = kind == 2 ? (double)P[i] : (double)P[i] - (double)J[i];
+            } else if (kind == 2) {
+                const double pold = (double)P[i] - (double)J[i];
+                P[i] = (float)u;
+                J[i] = (float)((double)P[i] - pold);
+            } else {
+                J[i] = (float)((double)P[i] - u);
+            }
+        }
+    if (write) {
+        HIPCHK(hipMemcpy(s.d_P, P.data(), P.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(s.d_Po, J.data(), J.size() * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
 static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     if (!x || !buf || mat < 0 || mat >= x->nmat || kind < 0 || kind > 3) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
     const int sp = x->special_of[mat];
     if (sp < 0) return 0;
     Special &s = x->specials[sp];
+    if (kind >= 2 && s.pre.ade && x->prec == PHPFDTD_FP32) return ade_state_f32(x, s, kind, buf, write);
     const size_t n = s.cells.size();
     int width;
     void *dev;

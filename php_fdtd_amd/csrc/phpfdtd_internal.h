@@ -187,9 +187,14 @@ struct SlotArgs {
     // crystal-frame coefficients
     real R[9];    // lab -> crystal rotation
     real A[9];    // E_cr = A . (D_cr - P)  (lf, 1-lt, Re(eps^-1), 1/eps0 folded)
-    // ADE (fdtd.c:3463-3479), P^{n+1} = Gpi (2P - Gm P^{n-1} + W E^n_cr)
+    // ADE (fdtd.c:3463-3479), P^{n+1} = Gpi (2P - Gm P^{n-1} + W E^n_cr).
+    // FP64 form: that recurrence on (P, P^{n-1}).  FP32 form: the same
+    // recurrence on (P, J = P - P^{n-1}): u = J - Ga J + W E, J' = u - Gb u,
+    // P' = P + J' (Ga = gamma dt/2, Gb = I - Gpi, both small), which keeps
+    // the marginal P mode exact in FP32.
     int ade;
     real Gpi[9], Gm[9], W[9];
+    real Ga[9], Gb[9];
     real *P, *Po;  // [3][nslot]
     // lattice (fdtd.c:3505-3528)
     int natom;

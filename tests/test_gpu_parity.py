@@ -303,3 +303,41 @@ def test_state_roundtrip():
     sim.set_field(5, f)
     np.testing.assert_array_equal(sim.field(5), f)
     sim.close()
+
+
+def test_fp32_ade_state_exchange():
+    """FP32 form ADE state (stored as P and J = P - P_old): P and P_old round
+    to FP32 on the way in, come back as written in either write order."""
+    c = golden_cases.load("ade")
+    spec = c["spec"]
+    m = [i for i, mt in enumerate(spec["materials"]) if mt["use_ade"] and i >= 4][0]
+    sim = Simulation(spec, c["medium"], _abi.FP32)
+    n = sim.count_cells(m)
+    rng = np.random.default_rng(5)
+    p, po = rng.standard_normal(3 * n) * 1e-11, rng.standard_normal(3 * n) * 1e-11
+    for order in ((2, 3), (3, 2)):
+        for kind in order:
+            sim.set_state(m, kind, p if kind == 2 else po)
+        np.testing.assert_allclose(sim.state(m, 2), p, rtol=2 ** -23)
+        np.testing.assert_allclose(sim.state(m, 3), po, rtol=0, atol=4 * 2 ** -24 * np.abs(p).max())
+    sim.close()
+
+
+def test_fp32_ade_long_run():
+    """A driven ADE slab over 10,000 steps: the FP32 form stays within 1e-4 of
+    the FP64 form (the increment form of the ADE recurrence; the reference's
+    two-level form in FP32 drifts to ~1e-2 here)."""
+    d = dict(id=0, shape=(64, 64, 64), kind="lorentz", wn=1244.8, steps=10000)
+    spec, med = workloads.serving_case(d)
+    assert spec["materials"][4]["use_ade"] == 1
+    rows = {}
+    for p in (_abi.FP64, _abi.FP32):
+        sim = Simulation(spec, med, p)
+        m = spec["monitor"]
+        sim.set_monitor(m["x"], m["y"], m["z"], m["atom"], m["dir"], ref_compat=True)
+        sim.run(d["steps"])
+        rows[p] = sim.monitor(d["steps"])
+        sim.close()
+    err = rel_l2(rows[_abi.FP32][:, 1:10], rows[_abi.FP64][:, 1:10])
+    print("ADE 10000 steps FP32 vs FP64 rel L2", err, flush=True)
+    assert err <= 1e-4
