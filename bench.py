@@ -12,7 +12,7 @@ library's stream, algorithmic bytes), the whole-step byte model, the FP32 vs
 FP64 monitor error over the timed run, and the reference solver on the host
 cores on a bounded sample of the same workload (cpu_baseline).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tip_slab]
 """
 import argparse
 import json
@@ -29,7 +29,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-PROFILE = os.path.join(ROOT, "profiles", "cfg3_step_kernels.md")
+PROFILE = os.path.join(ROOT, "profiles", "step_kernels_tip_slab_1024x1024x640.md")
 MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 METRIC = "Gcell-updates/s + HBM roofline fraction; solves/s, p50/p99, FP32 vs FP64"
@@ -145,7 +145,7 @@ def accuracy(name, interval=10, steps=None):
     from php_fdtd_amd import _abi, workloads
     from php_fdtd_amd.sim import Simulation
 
-    spec, med = workloads.CONFIGS[name]()
+    spec, med = workloads.WORKLOADS[name]()
     if steps:
         spec = workloads.with_steps(spec, steps)
     g = spec["grid"]
@@ -276,7 +276,7 @@ def cpu_baseline(budget_s=25.0):
     except Exception:
         ncpu = os.cpu_count() or 1
     cores = max(1, min(16, ncpu))
-    spec, med = workloads.config3(scale=4)
+    spec, med = workloads.tip_slab(scale=4)
     cells = med.size
     if os.path.exists(ref) and os.path.exists(mpirun):
         # probe 2 steps to size the sample
@@ -298,12 +298,12 @@ def cpu_baseline(budget_s=25.0):
             t = run(n)
             return dict(value=cells * n / t / 1e9, unit="Gcell-updates/s", cores=cores, kind="reference",
                         sample="reference fdtd.c (-Ofast -flto -funroll-loops) under mpirun -np %d, "
-                               "256x256x160 (config 3 scaled 1/4 per axis), %d steps, loop time %.2f s" % (cores, n, t))
+                               "256x256x160 (the 1024x1024x640 workload scaled 1/4 per axis), %d steps, loop time %.2f s" % (cores, n, t))
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
     from oracle import orc
 
-    spec, med = workloads.config3(scale=8)
+    spec, med = workloads.tip_slab(scale=8)
     spec = workloads.with_steps(spec, 1000)
     o = orc.Oracle(spec, med)
     t0 = time.time()
@@ -313,7 +313,7 @@ def cpu_baseline(budget_s=25.0):
         n += 1
     t = time.time() - t0
     return dict(value=med.size * n / t / 1e9, unit="Gcell-updates/s", cores=1, kind="port",
-                sample="C oracle (FP64 restatement), 1 thread, 128x128x80 (config 3 scaled 1/8), %d steps" % n)
+                sample="C oracle (FP64 restatement), 1 thread, 128x128x80 (the 1024x1024x640 workload scaled 1/8), %d steps" % n)
 
 
 def main():
@@ -321,18 +321,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--workload", "--config", dest="workload", default="tip_slab",
+                    help="tip_slab, lattice16_slab, lattice4_slab or lorentz_slab")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64", help="comma list of fp32, fp32rv64, fp64")
-    ap.add_argument("--accuracy", default="cfg1", help="config for the FP32-vs-FP64 accuracy run ('' = skip)")
-    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy run (0 = the config's own)")
+    ap.add_argument("--accuracy", default="lattice4_slab",
+                    help="workload of the FP32-vs-FP64 accuracy run ('' = skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy run (0 = the workload's own)")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving mix (0 = skip)")
     ap.add_argument("--serving-step-scale", type=float, default=0.05)
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false",
                     help="skip the measured copy-bandwidth ceilings")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the domain is resident on a single MI355X and no "
+        print("bench.py: runs on one GPU only; the domain is resident on a single MI355X and no "
               "multi-GPU path is built (--gpus %d not supported)" % args.gpus, file=sys.stderr)
         return 2
 
@@ -340,7 +342,7 @@ def main():
 
     prof_steps = 3
     t0 = time.time()
-    spec, med = workloads.CONFIGS[args.config]()
+    spec, med = workloads.WORKLOADS[args.workload]()
     spec = workloads.with_steps(spec, args.warmup + args.steps + prof_steps + max(1, args.steps // 10) * 10)
     ceiling = copy_ceiling() if args.copy_probe else None
     if ceiling:
@@ -382,16 +384,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": "%s: %dx%dx%d cells, 16-atom lattice slab + Drude cone, %s form (fields/CPML/ADE "
-                               "%s, lattice r,v %s)" % (spec["name"], spec["grid"]["nx"], spec["grid"]["ny"],
-                                                       spec["grid"]["nz"], head.upper(), "f64" if head == "fp64" else
-                                                       "f32", rv_desc),
+        "config": {"workload": "%s: 16-atom lattice slab + Drude cone, %s form (fields/CPML/ADE %s, lattice r,v "
+                               "%s)" % (spec["name"], head.upper(), "f64" if head == "fp64" else "f32", rv_desc),
                    "cells": int(cells), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic) if traffic else None,
                      "algorithmic_bytes": round(dom["bytes"]), "symbol": dom_sym,
-                     "traffic_source": "profiles/cfg3_step_kernels.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per "
+                     "traffic_source": "profiles/step_kernels_tip_slab_1024x1024x640.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per "
                                        "dispatch, copy-probe calibrated" if traffic else None},
         "step_roofline": {"bytes_per_step": sb, "achieved": round(sb / (r["ms"] * 1e-3) / 1e9, 1),
                           "frac": round(sb / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -4,24 +4,28 @@ plus writers of the reference's own input formats (cfg keys, medium_map.bin,
 QeZV/QeZM/DMM text, OUTCAR fragment, 15-column dielectric table) so the same
 workload can be run by the reference solver.
 
-Configurations (sizes as named):
-  cfg0  100^3, 20 nm, 2000 steps, 10-cell CPML; isotropic Lorentz slab
-        (eps_inf 2.5, w0 1370 cm^-1, d_eps 2.0, gamma 5 cm^-1) taken at the
-        source wavenumber 650 cm^-1 through the reference's Drude-fit ADE.
-  cfg1  160x160x120, 20 nm, 20000 steps, 12-cell CPML; 20-cell slab of a
-        4-atom lattice (10.8, 10.8, 14.0, 14.0 amu) rotated anglez = 30 deg.
-  cfg2  384x384x256, 20 nm, 30000 steps, 16-cell CPML; 32-cell slab of a
-        16-atom lattice (4 x 95.95 + 12 x 16.0 amu), anglex = 90 deg + switchxz,
-        full complex eps tensor.
-  cfg3  1024x1024x640, 10 nm, 10000 steps, 20-cell CPML; 48-cell slab of the
-        cfg2 material plus a Drude-metal cone (apex radius 10 cells, 30 deg
-        half-angle) 3 cells above the slab, Ricker source under the apex.
-  cfg4  serving mix: grids {64^3, 96^3, 128^3, 192x192x128, 256x256x160},
-        materials {cfg0-, cfg1-, cfg2-type}, source U(600, 1600) cm^-1,
-        steps U(5000, 20000), 10-cell CPML, Poisson arrivals.
+Workloads (key in WORKLOADS, spec name with its grid size):
+  lorentz_slab    100^3, 20 nm, 2000 steps, 10-cell CPML; isotropic Lorentz slab
+                  (eps_inf 2.5, w0 1370 cm^-1, d_eps 2.0, gamma 5 cm^-1) taken
+                  at the source wavenumber 650 cm^-1 through the reference's
+                  Drude-fit ADE.
+  lattice4_slab   160x160x120, 20 nm, 20000 steps, 12-cell CPML; 20-cell slab
+                  of a 4-atom lattice (10.8, 10.8, 14.0, 14.0 amu) rotated
+                  anglez = 30 deg.
+  lattice16_slab  384x384x256, 20 nm, 30000 steps, 16-cell CPML; 32-cell slab
+                  of a 16-atom lattice (4 x 95.95 + 12 x 16.0 amu), anglex =
+                  90 deg + switchxz, full complex eps tensor.
+  tip_slab        1024x1024x640, 10 nm, 10000 steps, 20-cell CPML; 48-cell slab
+                  of the 16-atom lattice material plus a Drude-metal cone (apex
+                  radius 10 cells, 30 deg half-angle) 3 cells above the slab,
+                  Ricker source under the apex.
+  serving mix     grids {64^3, 96^3, 128^3, 192x192x128, 256x256x160},
+                  materials {Lorentz, 4-atom lattice, 16-atom lattice}, source
+                  U(600, 1600) cm^-1, steps U(5000, 20000), 10-cell CPML,
+                  Poisson arrivals.
 
 Every medium is drawn by ``php_fdtd_amd.medium`` from a ``[medium]`` geometry
-(cubes, plus a cone for cfg3) kept in ``spec["medium"]``; the cfg written by
+(cubes, plus a cone for tip_slab) kept in ``spec["medium"]``; the cfg written by
 ``write_reference_case`` carries it, so the same map can be redrawn from the
 cfg alone.
 """
@@ -155,14 +159,14 @@ def _draw(spec, geometry):
     return _medium.build((g["nx"], g["ny"], g["nz"]), (g["dx"], g["dy"], g["dz"]), geometry)
 
 
-def material_cfg0(f):
+def material_lorentz(f):
     er, ei = lorentz_eps(f / WN_TO_HZ, 2.5, 1370.0, 2.0, 5.0)
     m = vacuum()
     m.update(eps_re=diag(er, er, er), eps_im=diag(ei, ei, ei), use_ade=1, name="lorentz650")
     return m
 
 
-def material_cfg1(seed=1):
+def material_lattice4(seed=1):
     lat = lattice(4, [10.8, 10.8, 14.0, 14.0], 34.626, 750.0, 1650.0, 2.7, seed)
     m = vacuum()
     m.update(eps_re=diag(4.9, 4.9, 2.9), anglez=math.radians(30.0), natom=4, qezv=lat["qezv"], qezm=lat["qezm"],
@@ -171,7 +175,7 @@ def material_cfg1(seed=1):
     return m
 
 
-def material_cfg2(seed=2):
+def material_lattice16(seed=2):
     lat = lattice(16, [95.95] * 4 + [16.0] * 12, 200.536, 200.0, 1050.0, 2.5, seed)
     rng = np.random.default_rng(seed + 100)
     re = np.diag(rng.uniform(2, 6, 3))
@@ -188,30 +192,30 @@ def material_cfg2(seed=2):
     return m
 
 
-def config0():
+def lorentz_slab():
     f = 650.0 * WN_TO_HZ
     g = _grid((100, 100, 100), 20e-9, 2000, 10, f)
-    mats = builtin_table(f) + [material_cfg0(f)]
+    mats = builtin_table(f) + [material_lorentz(f)]
     src = [_source("ricker", (50, 50, 64), "z", f, t0=5.128e-14, tsteps=2000, dt=g["dt"])]
-    spec = dict(name="cfg0", grid=g, materials=mats, sources=src, monitor=dict(x=50, y=50, z=69, atom=0, dir=2))
+    spec = dict(name="lorentz_slab_100x100x100", grid=g, materials=mats, sources=src, monitor=dict(x=50, y=50, z=69, atom=0, dir=2))
     return spec, _draw(spec, dict(cubes=[_slab((100, 100, 100), 40, 60)]))
 
 
-def config1():
+def lattice4_slab():
     f = 1400.0 * WN_TO_HZ
     g = _grid((160, 160, 120), 20e-9, 20000, 12, f)
-    mats = builtin_table(f) + [material_cfg1()]
+    mats = builtin_table(f) + [material_lattice4()]
     src = [_source("dipole", (80, 80, 45), "z", f, tsteps=20000, dt=g["dt"])]
-    spec = dict(name="cfg1", grid=g, materials=mats, sources=src, monitor=dict(x=80, y=80, z=60, atom=0, dir=0))
+    spec = dict(name="lattice4_slab_160x160x120", grid=g, materials=mats, sources=src, monitor=dict(x=80, y=80, z=60, atom=0, dir=0))
     return spec, _draw(spec, dict(cubes=[_slab((160, 160, 120), 50, 70)]))
 
 
-def config2():
+def lattice16_slab():
     f = 800.0 * WN_TO_HZ
     g = _grid((384, 384, 256), 20e-9, 30000, 16, f)
-    mats = builtin_table(f) + [material_cfg2()]
+    mats = builtin_table(f) + [material_lattice16()]
     src = [_source("dipole", (192, 192, 100), "z", f, tsteps=30000, dt=g["dt"])]
-    spec = dict(name="cfg2", grid=g, materials=mats, sources=src, monitor=dict(x=192, y=192, z=128, atom=0, dir=0))
+    spec = dict(name="lattice16_slab_384x384x256", grid=g, materials=mats, sources=src, monitor=dict(x=192, y=192, z=128, atom=0, dir=0))
     return spec, _draw(spec, dict(cubes=[_slab((384, 384, 256), 112, 144)]))
 
 
@@ -232,7 +236,7 @@ def material_drude_tip(f):
     return m
 
 
-def config3(scale=1):
+def tip_slab(scale=1):
     """scale > 1 shrinks every length by that factor (same physics layout);
     scale=1 is the full 1024x1024x640 configuration."""
     f = 2.7e13
@@ -242,19 +246,20 @@ def config3(scale=1):
     slab0 = n[2] * 200 // 640
     slab1 = slab0 + max(2, 48 // scale)
     apex = slab1 + max(1, 3 // scale)
-    mats = builtin_table(f) + [material_cfg2(), material_drude_tip(f)]
+    mats = builtin_table(f) + [material_lattice16(), material_drude_tip(f)]
     src_z = slab1 + max(0, (apex - slab1) // 2)
     src = [_source("ricker", (n[0] // 2, n[1] // 2, src_z), "z", f, t0=5e-14, tsteps=10000, dt=g["dt"])]
-    spec = dict(name="cfg3" if scale == 1 else "cfg3/%d" % scale, grid=g, materials=mats, sources=src,
+    spec = dict(name="tip_slab_%dx%dx%d" % n, grid=g, materials=mats, sources=src,
                 monitor=dict(x=n[0] // 2, y=n[1] // 2, z=slab0 + (slab1 - slab0) // 2, atom=0, dir=2))
     geometry = dict(cubes=[_slab(n, slab0, slab1)],
                     cones=[tip_cone(n, apex, max(1, 10 // scale), 30.0, n[2] - L, g["dx"])])
     return spec, _draw(spec, geometry)
 
 
-CONFIGS = {"cfg0": config0, "cfg1": config1, "cfg2": config2, "cfg3": config3}
+WORKLOADS = {"lorentz_slab": lorentz_slab, "lattice4_slab": lattice4_slab, "lattice16_slab": lattice16_slab,
+             "tip_slab": tip_slab}
 
-# serving mix (cfg4)
+# serving mix
 SERVE_GRIDS = [(64, 64, 64), (96, 96, 96), (128, 128, 128), (192, 192, 128), (256, 256, 160)]
 SERVE_KINDS = ("lorentz", "lattice4", "lattice16")
 
@@ -290,7 +295,7 @@ def serving_case(req):
     s1 = nz * 7 // 12
     kind = req["kind"]
     if kind == "lorentz":
-        mat = material_cfg0(f)
+        mat = material_lorentz(f)
         # the reference's Drude-fit ADE needs Re eps < EPS_INF = 20
         # (fdtd.c:2478-2485); nearer the 1370 cm^-1 pole the slab runs as a
         # lossy dielectric (eps_im loss term), as a reference user would set it
@@ -298,7 +303,7 @@ def serving_case(req):
             mat = dict(mat, use_ade=0)
     else:
         if kind not in _LATTICE_CACHE:
-            _LATTICE_CACHE[kind] = material_cfg1() if kind == "lattice4" else material_cfg2()
+            _LATTICE_CACHE[kind] = material_lattice4() if kind == "lattice4" else material_lattice16()
         mat = _LATTICE_CACHE[kind]
     src = [_source("dipole", (nx // 2, ny // 2, s0 - 5), "z", f, tsteps=req["steps"], dt=g["dt"])]
     spec = dict(name="req%d/%s/%dx%dx%d" % (req["id"], kind, nx, ny, nz), grid=g, materials=builtin_table(f) + [mat],

@@ -1,4 +1,4 @@
-"""Size-independent properties at the full 1024x1024x640 size (config 3).
+"""Size-independent properties at the full 1024x1024x640 size (tip_slab workload).
 
 The update is linear in the source amplitude and scaling by 2 is exact in
 binary floating point, so doubling every source amplitude doubles every
@@ -41,8 +41,8 @@ def sample(spec, med, precision, scale):
 
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32])
 def test_full_size_linearity(precision):
-    spec, med = workloads.config3()
-    print("config 3 full size, precision", precision, flush=True)
+    spec, med = workloads.tip_slab()
+    print("tip_slab 1024x1024x640, precision", precision, flush=True)
     a = sample(spec, med, precision, 1.0)
     b = sample(spec, med, precision, 2.0)
     assert np.abs(a["slice_z_Ez"]).max() > 0 and np.abs(a["max_e"]).max() > 0
@@ -66,7 +66,7 @@ def test_full_size_linearity(precision):
 
 
 def test_no_sources_stays_zero():
-    spec, med = workloads.config1()
+    spec, med = workloads.lattice4_slab()
     spec = dict(workloads.with_steps(spec, 12), sources=[])
     for precision in (_abi.FP64, _abi.FP32):
         sim = Simulation(spec, med, precision)

@@ -248,7 +248,7 @@ def test_compensated_vs_fp64_state(name):
     assert abs(a[2].max_r - b[2].max_r) <= 1e-5 * max(b[2].max_r, 1e-300)
 
 
-@pytest.mark.parametrize("case", ["hbn", "cfg3/8"])
+@pytest.mark.parametrize("case", ["hbn", "tip_slab/8"])
 def test_graph_matches_stream_launches(monkeypatch, case):
     """16-step graphs (with the next step's lattice-free H ranges on a second
     stream beside the lattice kernel) give the same bits as plain sequential
@@ -257,7 +257,7 @@ def test_graph_matches_stream_launches(monkeypatch, case):
         c = golden_cases.load("hbn")
         spec, med, precs = c["spec"], c["medium"], (_abi.FP64,)
     else:  # 1024x1024x640 layout at 1/8 scale: lattice slab + Drude ADE cone
-        spec, med = workloads.config3(8)
+        spec, med = workloads.tip_slab(8)
         spec, precs = workloads.with_steps(spec, 40), (_abi.FP64, _abi.FP32)
     for prec in precs:
         res = []

@@ -336,7 +336,7 @@ def test_format_section_round_trip():
 # workloads, command line
 # --------------------------------------------------------------------------
 def test_workload_slabs():
-    for build, n, z0, z1 in ((W.config0, (100, 100, 100), 40, 60), (W.config1, (160, 160, 120), 50, 70)):
+    for build, n, z0, z1 in ((W.lorentz_slab, (100, 100, 100), 40, 60), (W.lattice4_slab, (160, 160, 120), 50, 70)):
         spec, med = build()
         ref = np.zeros(n, np.int32)
         ref[:, :, z0:z1] = 4
@@ -348,7 +348,7 @@ def test_tip_cone_per_plane():
     """The 1024x1024x640 tip at 1/8 scale: on plane z the tip is the disc
     (x - cx)^2 + (y - cy)^2 <= (r0 + (z - apex) tan 30deg)^2 in cells; cells
     exactly on a circle (integer radius at the apex) may go either way."""
-    spec, med = W.config3(8)
+    spec, med = W.tip_slab(8)
     cone = spec["medium"]["cones"][0]
     cx, cy, apex = (int(c) for c in cone["p1"])
     top = int(cone["p2"][2]) + 1
@@ -387,6 +387,6 @@ def test_command_line(tmp_path):
 
 
 def test_reference_case_redraws_its_medium(tmp_path):
-    spec, med = W.config0()
+    spec, med = W.lorentz_slab()
     path = W.write_reference_case(str(tmp_path / "c0"), spec, med, with_born=False)
     assert np.array_equal(M.read_config(path), M.load_medium_file(str(tmp_path / "c0" / "medium_map.bin")))
