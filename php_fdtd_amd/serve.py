@@ -183,6 +183,12 @@ class Server:
                 f.result()
         finally:
             pool.shutdown(wait=True)
+            # after an error: release the contexts still held (a normal end holds none)
+            for r, fut in building:
+                if not fut.exception():
+                    fut.result().close()
+            for r, sim, _ in active:
+                sim.close()
             batch.close()
         return reqs
 
