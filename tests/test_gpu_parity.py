@@ -206,6 +206,62 @@ def test_lattice_atom_counts(natom):
         s.close()
 
 
+def ragged_case(shape):
+    """A grid whose x, y and z extents end inside a wavefront tile, a y block
+    and a z chunk of both forms (FP32: 256-cell x tiles, 4-plane chunks;
+    FP64: 128-cell x tiles, 2-row waves, 16-plane chunks), with a 3-cell CPML
+    on every face, a rotated 4-atom lattice slab and a Drude ADE block that
+    straddle the x tile seams, a lossy dielectric block, and Gaussian sources
+    (peak at t = 0, so they inject in the first steps) on both sides of each
+    seam."""
+    nx, ny, nz = shape
+    f = 1000.0 * workloads.WN_TO_HZ
+    g = workloads._grid(shape, 20e-9, 40, 3, f)
+    lossy = dict(workloads.material_lorentz(f), use_ade=0)
+    mats = workloads.builtin_table(f) + [workloads.material_lattice4(), workloads.material_drude_tip(f), lossy]
+    med = np.zeros(shape, np.int32)
+    med[nx // 3:nx - 5, 2:ny - 2, 8:12] = 4
+    med[100:nx - 10, 1:ny - 1, 14:18] = 5
+    med[4:60, 3:ny - 3, 2:7] = 6
+    amp = 1e-11 / g["dt"]  # D changes by ~1e-11, the size of the random D state
+    src = [workloads._source("gaussian", (x, ny // 2, z), c, f, t0=0.0, amplitude=amp, tsteps=40, dt=g["dt"])
+           for x, z, c in ((127, 5, "x"), (128, 13, "y"), (255, 19, "z"), (256, 10, "x"), (nx - 2, 16, "z"))]
+    spec = dict(name="ragged%dx%dx%d" % shape, grid=g, materials=mats, sources=src)
+    return spec, med
+
+
+@pytest.mark.parametrize("shape", [(300, 13, 21), (257, 9, 18)])
+def test_ragged_tiles_vs_oracle(shape):
+    """Every field, lattice and ADE state of both forms against the oracle
+    from a random state on grids with ragged x tiles, y blocks and z chunks;
+    the lattice slab also splits the H update into three z ranges."""
+    spec, med = ragged_case(shape)
+    print("ragged case", shape, flush=True)
+    o = orc.Oracle(spec, med)
+    for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL)):
+        sim = Simulation(spec, med, prec)
+        oo = orc.Oracle(spec, med) if prec != _abi.FP64 else o
+        randomize(spec, med, 7, sim, oo)
+        if prec == _abi.FP64:
+            o.step(6)
+        sim.run(6)
+        for comp in range(9):
+            e = rel_l2(sim.field(comp), o.field(comp))
+            assert e <= tol, "precision %d component %d rel err %g" % (prec, comp, e)
+        for m, kinds in ((4, (0, 1)), (5, (2, 3))):
+            assert sim.count_cells(m) == int((med == m).sum())
+            for kind in kinds:
+                e = rel_l2(sim.state(m, kind), o.state(m, kind))
+                assert e <= tol, "precision %d material %d kind %d rel err %g" % (prec, m, kind, e)
+        if prec == _abi.FP64:
+            d = sim.sync()
+            v, idx = o.diag()
+            assert abs(d.max_e - v[0]) <= 1e-12 * v[0]
+            assert abs(d.max_r - v[2]) <= 1e-12 * v[2]
+            assert [d.r_i, d.r_j, d.r_k, d.r_atom, d.r_dir] == idx
+        sim.close()
+
+
 def test_compensated_state():
     """Lattice r, v of the FP32 form are stored as an FP32 word plus an 8-bit
     compensation term: any double written comes back within 2^-31 of itself
