@@ -93,13 +93,14 @@ __global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, 
 }
 
 // max |r| over one lattice material's state (FP64 values, or the FP32 words
-// of compensated state: max |r| rounded to FP32; its compensation words,
-// the last kPkGroup - kPkWords of every kPkGroup, are skipped).
+// of compensated state: max |r| rounded to FP32; the compensation rows, the
+// elements of a tile from `words` on, are skipped).
 template <typename T>
-__global__ __launch_bounds__(256) void reduce_r(const T *r, int64_t n, unsigned long long *out) {
+__global__ __launch_bounds__(256) void reduce_r(const T *r, int64_t n, int64_t tile, int64_t words,
+                                                unsigned long long *out) {
     unsigned long long m = 0;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        if (sizeof(T) == 4 && q % kPkGroup >= kPkWords) continue;
+        if (q % tile >= words) continue;
         const unsigned long long b = __double_as_longlong(fabs((double)r[q]));
         m = m > b ? m : b;
     }
@@ -225,11 +226,16 @@ void launch_reduce_e(const real *Ex, const real *Ey, const real *Ez, int nx, int
     hipLaunchKernelGGL(reduce_e<real>, dim3(2048), dim3(256), 0, s, Ex, Ey, Ez, nx, ny, nz, nxp, out);
 }
 
-// r: double values, or float words when f32 (compensated state).
-void launch_reduce_r(const void *r, bool f32, int64_t n, unsigned long long *out, hipStream_t s) {
+// r: ntile tiles of layout lay; double values, or float words when f32 (compensated state).
+void launch_reduce_r(const void *r, bool f32, int64_t ntile, IonLayout lay, unsigned long long *out, hipStream_t s) {
+    const int64_t tile = lay.tile_r(), n = ntile * tile;
     if (n <= 0) return;
-    if (f32) hipLaunchKernelGGL(reduce_r<float>, dim3(1024), dim3(256), 0, s, static_cast<const float *>(r), n, out);
-    else hipLaunchKernelGGL(reduce_r<double>, dim3(1024), dim3(256), 0, s, static_cast<const double *>(r), n, out);
+    if (f32)
+        hipLaunchKernelGGL(reduce_r<float>, dim3(1024), dim3(256), 0, s, static_cast<const float *>(r), n, tile,
+                           (int64_t)kPkWords * lay.ng(), out);
+    else
+        hipLaunchKernelGGL(reduce_r<double>, dim3(1024), dim3(256), 0, s, static_cast<const double *>(r), n, tile,
+                           tile, out);
 }
 
 void launch_argmax_r(const void *r, bool f32, const uint32_t *cells, int64_t nslot, IonLayout lay, int ny, int nz,

@@ -226,20 +226,22 @@ __host__ __device__ constexpr int mfma_wave_lds(int na) {
                                                                          : (3 * na + 15) / 16 * 16 * kRowA;
 }
 
-// Compensated FP32 state (IonLayout w = 4): group j of a tile is kPkGroup
-// words, the lanes' FP32 words at word 4 lane, their compensation bytes at
-// word kPkWords + lane.  `base` is the tile's first word.
+// Compensated FP32 state (IonLayout w = 4): `base` is the tile's first word;
+// group j holds the lanes' FP32 words at word 256 j + 4 lane, compensation
+// row b (after the ng groups) the lane's dwords of groups 4b .. 4b+3 at word
+// 256 (ng + b) + 4 lane.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ f32x4 ldquad(const uint32_t *base, int lane, int j) {
-    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * kPkGroup + 4 * lane));
+    return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * kPkWords + 4 * lane));
 }
-__device__ __forceinline__ uint32_t ldbytes(const uint32_t *base, int lane, int j) {
-    return __builtin_nontemporal_load(base + (int64_t)j * kPkGroup + kPkWords + lane);
+__device__ __forceinline__ u32x4 ldcomp(const uint32_t *base, int lane, int ng, int b) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(base + (int64_t)(ng + b) * kPkWords + 4 * lane));
 }
 __device__ __forceinline__ void stquad(f32x4 v, uint32_t *base, int lane, int j) {
-    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * kPkGroup + 4 * lane));
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(base + (int64_t)j * kPkWords + 4 * lane));
 }
-__device__ __forceinline__ void stbytes(uint32_t v, uint32_t *base, int lane, int j) {
-    __builtin_nontemporal_store(v, base + (int64_t)j * kPkGroup + kPkWords + lane);
+__device__ __forceinline__ void stcomp(u32x4 v, uint32_t *base, int lane, int ng, int b) {
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(base + (int64_t)(ng + b) * kPkWords + 4 * lane));
 }
 
 // Byte h of a packed dword as a signed value, and its replacement.
@@ -258,10 +260,12 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     constexpr int WL = mfma_wave_lds(NA);
     constexpr int W = PK ? 4 : 2;  // dofs per lane vector (IonLayout w)
     constexpr int NVX = (N3X + W - 1) / W;
+    constexpr int NBX = (NVX + 3) / 4;  // compensation rows (compensated state)
     const int natom = exact ? NA : a.natom;
     const int N3 = 3 * natom;
     const int NK = exact ? NKX : (N3 + 3) / 4, NMA = exact ? NAX : (N3 + 15) / 16;
     const int NV = (N3 + W - 1) / W;
+    const int NB = (NV + 3) / 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t tile = blk * WPB + wave;
@@ -278,11 +282,16 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     float *buf = lds_a + wave * WL;
     // FP64 state: dof group j of this lane's slot at tb + 128 j; compensated: tile words at tw
     const int64_t tb = tile * 64 * W * (int64_t)NV + W * lane;
-    const int64_t tw = tile * kPkGroup * (int64_t)NV;
-    // r^n: FP64 values (FP64 state), or FP32 words + compensation dwords
+    const int64_t tw = tile * (int64_t)kPkWords * (NV + NB);
+    // r^n: FP64 values (FP64 state), or FP32 words + compensation rows
     double r[PK ? 1 : W * NVX];
     float rw[PK ? W * NVX : 1];
-    uint32_t rc[PK ? NVX : 1];
+    u32x4 rc[PK ? NBX : 1];
+    if constexpr (PK) {
+#pragma unroll
+        for (int b = 0; b < NBX; ++b)
+            if (b < NB) rc[b] = ldcomp(static_cast<const uint32_t *>(rr_) + tw, lane, NV, b);
+    }
 #pragma unroll
     for (int j = 0; j < NVX; ++j)
         if (j < NV) {
@@ -290,7 +299,6 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                 const uint32_t *rb = static_cast<const uint32_t *>(rr_) + tw;
                 const f32x4 h4 = ldquad(rb, lane, j);
                 for (int h = 0; h < 4; ++h) rw[4 * j + h] = h4[h];
-                rc[j] = ldbytes(rb, lane, j);
             } else {
                 const f64x2 v2 = ldpair(static_cast<const double *>(rr_) + tb, j);
                 r[2 * j] = v2.x;
@@ -382,13 +390,15 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     const double dt = a.dt, hdt2 = a.half_dt2;
     const float *qm = qop + 9 * natom;
     // FP64 integration r += v dt + a dt^2/2, v += a dt; dofs past 3N (padding) keep their zeros
+    u32x4 vcb;  // compensation row of v holding group j
 #pragma unroll
     for (int j = 0; j < NVX; ++j) {
         if (j >= NV) break;
         if constexpr (PK) {
             uint32_t *rb = static_cast<uint32_t *>(rr_) + tw, *vb = static_cast<uint32_t *>(vv_) + tw;
+            if (j % 4 == 0) vcb = ldcomp(vb, lane, NV, j / 4);
             f32x4 v4 = ldquad(vb, lane, j), r4;
-            uint32_t vc = ldbytes(vb, lane, j), rcn = rc[j];
+            uint32_t vc = vcb[j % 4], rcn = rc[j / 4][j % 4];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const int q = 4 * j + h;
@@ -409,9 +419,13 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                 vc = with_byte(vc, h, lo);
             }
             stquad(r4, rb, lane, j);
-            stbytes(rcn, rb, lane, j);
             stquad(v4, vb, lane, j);
-            stbytes(vc, vb, lane, j);
+            rc[j / 4][j % 4] = rcn;
+            vcb[j % 4] = vc;
+            if (j % 4 == 3 || j == NV - 1) {  // row complete (padding dwords stored as loaded)
+                stcomp(rc[j / 4], rb, lane, NV, j / 4);
+                stcomp(vcb, vb, lane, NV, j / 4);
+            }
         } else {
             double *rd = static_cast<double *>(rr_) + tb, *vd = static_cast<double *>(vv_) + tb;
             f64x2 v2 = ldpair(vd, j);

@@ -39,7 +39,7 @@ template <typename T> void launch_fill(T *, T, int64_t, hipStream_t);
 template <typename real>
 void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
                      hipStream_t);
-void launch_reduce_r(const void *, bool, int64_t, unsigned long long *, hipStream_t);
+void launch_reduce_r(const void *, bool, int64_t, IonLayout, unsigned long long *, hipStream_t);
 void launch_argmax_r(const void *, bool, const uint32_t *, int64_t, IonLayout, int, int, int, unsigned long long *,
                      hipStream_t);
 template <typename real>
@@ -172,8 +172,8 @@ struct phpfdtd_ctx {
         if (!stream && allocs.empty()) return;  // failed before touching the device
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
-        if (g1) hipGraphExecDestroy(g1);
-        if (g16) hipGraphExecDestroy(g16);
+        if (g1) (void)hipGraphExecDestroy(g1);
+        if (g16) (void)hipGraphExecDestroy(g16);
         if (ev) (void)hipEventDestroy(ev);
         for (hipEvent_t e : cap_events) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
@@ -220,7 +220,7 @@ struct phpfdtd_ctx {
         auto it = std::find(allocs.begin(), allocs.end(), p);
         if (it == allocs.end()) return;
         allocs.erase(it);
-        hipFree(p);
+        (void)hipFree(p);
     }
 };
 
@@ -517,8 +517,8 @@ static void build_plan(phpfdtd_ctx *x) {
 static void touch(phpfdtd_ctx *x);
 
 static int rebuild(phpfdtd_ctx *x) {
-    if (x->g1) { hipGraphExecDestroy(x->g1); x->g1 = nullptr; }
-    if (x->g16) { hipGraphExecDestroy(x->g16); x->g16 = nullptr; }
+    if (x->g1) { (void)hipGraphExecDestroy(x->g1); x->g1 = nullptr; }
+    if (x->g16) { (void)hipGraphExecDestroy(x->g16); x->g16 = nullptr; }
     build_plan(x);
     x->dirty = false;
     if (!x->use_graph) return 0;
@@ -561,7 +561,7 @@ static int rebuild(phpfdtd_ctx *x) {
         HIPCHK(hipStreamEndCapture(x->stream, &graph));
         hipGraphExec_t ex = nullptr;
         hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(PHPFDTD_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
         (which == 0 ? x->g1 : x->g16) = ex;
     }
@@ -802,7 +802,7 @@ int phpfdtd_set_sources(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
     auto it = std::find(x->allocs.begin(), x->allocs.end(), old);
     if (it != x->allocs.end()) {
         x->allocs.erase(it);
-        hipFree(old);
+        (void)hipFree(old);
     }
     std::vector<int> pos(4 * kMaxSrc, 0);
     for (int s = 0; s < na; ++s) {
@@ -1003,8 +1003,7 @@ int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
                                x->d_diag, x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
-            launch_reduce_r(sp.d_r, x->pk, (int64_t)(sp.cells.size() + 63) / 64 * sp.lay.tile_r(), x->d_diag,
-                            x->stream);
+            launch_reduce_r(sp.d_r, x->pk, (int64_t)(sp.cells.size() + 63) / 64, sp.lay, x->d_diag, x->stream);
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
             launch_argmax_r(sp.d_r, x->pk, sp.d_cells, (int64_t)sp.cells.size(), sp.lay, x->ny, x->nz, x->nxp,
@@ -1318,7 +1317,7 @@ int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out,
         o.launches = 1;
         *n_out = (int)k + 1;
     }
-    for (auto &e : ev) hipEventDestroy(e);
+    for (auto &e : ev) (void)hipEventDestroy(e);
     return 0;
 }
 
@@ -1341,8 +1340,8 @@ int phpfdtd_time_steps(phpfdtd_ctx *x, int nsteps, double *ms) {
     HIPCHK(hipEventSynchronize(b));
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     *ms = t / nsteps;
     return 0;
 }

@@ -100,37 +100,38 @@ inline int mfma_inst(int natom) {
 // zero dofs.
 //   * FP64 state (plain form, and the precision form with FP64 r, v): w = 2,
 //     r[tile][dof / 2][64][2] doubles.
-//   * compensated FP32 state (precision form): w = 4; group j of a tile is
-//     1280 bytes: the 64 lanes' four FP32 words (1 KiB), then the 64 lanes'
-//     four compensation bytes (one dword per lane, 256 B).  Words and bytes of
-//     an array stream from one address range.
-// Compensated group: 256 FP32 words (64 lanes x 4 dofs), then 64 words of
-// compensation bytes (one per lane).
-constexpr int kPkWords = 256, kPkGroup = 320;
+//   * compensated FP32 state (precision form): w = 4; a tile is its ng groups
+//     of 1 KiB (the 64 lanes' four FP32 words), then ceil(ng / 4) blocks of
+//     1 KiB of compensation bytes: word 4 lane + h of block b holds the four
+//     compensation bytes of group 4b + h of that lane, so the bytes also move
+//     as one 16-byte vector per lane.  Words and bytes of an array stream from
+//     one address range.
+constexpr int kPkWords = 256;  // 4-byte words per 1 KiB row
 
 struct IonLayout {
     int n3;
     int w;  // dofs per lane vector: 2 (FP64) or 4 (compensated FP32)
     __host__ __device__ int n3p() const { return (n3 + w - 1) / w * w; }
-    // elements per tile: doubles (FP64), or 4-byte words (compensated: 80 per
-    // padded dof, 64 FP32 words and 16 words of compensation bytes)
-    __host__ __device__ int64_t tile_r() const { return (w == 4 ? kPkGroup / 4LL : 64LL) * n3p(); }
+    __host__ __device__ int ng() const { return n3p() / w; }    // dof groups (1 KiB rows) per tile
+    __host__ __device__ int nb() const { return (ng() + 3) / 4; }  // compensation rows (w = 4)
+    // elements per tile: doubles (FP64), or 4-byte words (compensated)
+    __host__ __device__ int64_t tile_r() const { return w == 4 ? (int64_t)kPkWords * (ng() + nb()) : 64LL * n3p(); }
     __host__ __device__ int64_t tile_v() const { return tile_r(); }
     // element of dof q of slot s: the double, or the FP32 word
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
-        return (s >> 6) * tile_r() + (int64_t)(q / w) * (w == 4 ? kPkGroup : 128) + (s & 63) * w + q % w;
+        return (s >> 6) * tile_r() + (int64_t)(q / w) * 64 * w + (s & 63) * w + q % w;
     }
     __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
     // byte offset of the compensation byte of dof q of slot s (compensated state)
     __host__ __device__ int64_t c_index(int64_t s, int q) const {
-        return ((s >> 6) * tile_r() + (int64_t)(q / 4) * kPkGroup + kPkWords + (s & 63)) * 4 + q % 4;
+        return ((s >> 6) * tile_r() + (int64_t)kPkWords * (ng() + q / 16) + (s & 63) * 4 + (q / 4) % 4) * 4 + q % 4;
     }
     // slot and dof of element idx; false for padding and compensation words
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
         const int64_t t = idx / tile_r(), rem = idx % tile_r();
-        const int gsz = w == 4 ? kPkGroup : 128;
+        const int gsz = 64 * w;
+        if (rem >= (int64_t)gsz * ng()) return false;
         const int u = (int)(rem % gsz);
-        if (u >= 64 * w) return false;
         *q = (int)(rem / gsz) * w + u % w;
         *s = t * 64 + u / w;
         return *q < n3 && *s < nslot;
