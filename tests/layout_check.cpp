@@ -1,0 +1,131 @@
+// Host check of the lattice-state layout helpers (IonLayout) and of the
+// compensated FP32 value format (pk_pack / pk_unpack) in
+// php_fdtd_amd/csrc/phpfdtd_internal.h.  Built and run by test_layout.py;
+// prints "ok" or the first violated property and exits non-zero.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <random>
+#include <set>
+#include <vector>
+
+#include "../php_fdtd_amd/csrc/phpfdtd_internal.h"
+
+using namespace phpfdtd;
+
+#define CHECK(c, ...)                          \
+    do {                                       \
+        if (!(c)) {                            \
+            printf("FAIL %s: ", #c);           \
+            printf(__VA_ARGS__);               \
+            printf("\n");                      \
+            exit(1);                           \
+        }                                      \
+    } while (0)
+
+static void check_layout(int natom, bool packed) {
+    const IonLayout l = ion_layout(natom, packed);
+    const int n3 = 3 * natom, w = packed ? 4 : 2;
+    const int64_t es = packed ? 4 : 8;  // bytes per element
+    const int64_t tile = l.tile_r();
+    CHECK(l.n3p() % w == 0 && l.n3p() >= n3 && l.n3p() < n3 + w, "natom %d", natom);
+    CHECK(l.ng() * w == l.n3p(), "natom %d", natom);
+    if (packed) {
+        // words of ng groups, then ceil(ng/4) rows of compensation bytes, 1 KiB each
+        CHECK(tile * es == 1024LL * (l.ng() + (l.ng() + 3) / 4), "natom %d tile %lld", natom, (long long)tile);
+    } else {
+        CHECK(tile * es == 64LL * 8 * l.n3p(), "natom %d tile %lld", natom, (long long)tile);
+    }
+    std::set<int64_t> words, bytes;
+    for (int t = 0; t < 2; ++t)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int q = 0; q < l.n3p(); ++q) {
+                const int64_t s = 64LL * t + lane;
+                const int64_t ri = l.r_index(s, q);
+                CHECK(ri >= t * tile && ri < (t + 1) * tile, "natom %d slot %lld dof %d", natom, (long long)s, q);
+                CHECK(l.v_index(s, q) == ri, "natom %d", natom);
+                CHECK(words.insert(ri).second, "natom %d: element %lld used twice", natom, (long long)ri);
+                // a lane's w dofs of one group are one 16-byte vector, the 64 lanes' vectors one 1 KiB row
+                CHECK((ri - t * tile) * es == 1024LL * (q / w) + 16LL * lane + es * (q % w), "natom %d slot %lld dof %d",
+                      natom, (long long)s, q);
+                int64_t s2 = -1;
+                int q2 = -1;
+                const bool live = l.r_decode(ri, 1 << 20, &s2, &q2);
+                CHECK(live == (q < n3), "natom %d dof %d: decode says %d", natom, q, (int)live);
+                if (live) CHECK(s2 == s && q2 == q, "natom %d: decode(%lld) = (%lld, %d)", natom, (long long)ri,
+                                (long long)s2, q2);
+                CHECK(!l.r_decode(ri, s, &s2, &q2), "natom %d: slot %lld past nslot decoded", natom, (long long)s);
+                if (packed) {
+                    const int64_t ci = l.c_index(s, q);
+                    const int64_t cw = ci / 4;
+                    // compensation bytes sit in the rows after the ng word rows of the same tile
+                    CHECK(cw >= t * tile + (int64_t)kPkWords * l.ng() && cw < (t + 1) * tile,
+                          "natom %d slot %lld dof %d byte %lld", natom, (long long)s, q, (long long)ci);
+                    // one 16-byte vector per lane and row: the lane's bytes of groups 4b .. 4b+3
+                    CHECK((cw - t * tile - (int64_t)kPkWords * l.ng()) * 4 == 1024LL * (q / 16) + 16LL * lane + 4 * ((q / 4) % 4) &&
+                              ci % 4 == q % 4,
+                          "natom %d slot %lld dof %d", natom, (long long)s, q);
+                    CHECK(bytes.insert(ci).second, "natom %d: byte %lld used twice", natom, (long long)ci);
+                }
+            }
+    if (packed) {
+        // every element of the compensation rows decodes as no dof
+        for (int64_t e = (int64_t)kPkWords * l.ng(); e < tile; ++e) {
+            int64_t s2;
+            int q2;
+            CHECK(!l.r_decode(e, 1 << 20, &s2, &q2), "natom %d: compensation element %lld decoded", natom, (long long)e);
+        }
+        CHECK((int64_t)words.size() * 4 + (int64_t)bytes.size() <= 2 * tile * 4, "natom %d", natom);
+    } else {
+        CHECK((int64_t)words.size() == 2 * tile, "natom %d: FP64 tiles not covered", natom);
+    }
+}
+
+static void check_values() {
+    std::mt19937_64 g(5);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    std::uniform_int_distribution<int> ex(-140, 120);
+    // zero is exact
+    float hi;
+    int lo;
+    pk_pack(0.0, &hi, &lo);
+    CHECK(hi == 0.f && lo == 0 && pk_unpack(hi, lo) == 0.0, "zero");
+    for (int n = 0; n < 200000; ++n) {
+        const double x = ldexp(u(g), ex(g));
+        pk_pack(x, &hi, &lo);
+        CHECK(lo >= -127 && lo <= 127, "x %.17g lo %d", x, lo);
+        CHECK(hi == (float)x, "x %.17g: word is not fl32(x)", x);
+        const double y = pk_unpack(hi, lo);
+        // within half a unit (2^-8 ulp of the word) where the correction fits
+        // the byte, one unit where it is clamped to +-127
+        const int e = pk_exp(hi);
+        const double unit = ldexp(1.0, e - 158);
+        const bool clamped = lo == 127 || lo == -127;
+        CHECK(fabs(y - x) <= (clamped ? 1.0 : 0.5) * unit, "x %.17g y %.17g unit %g lo %d", x, y, unit, lo);
+        // relative, normal words: 32 significant bits (2^-32), 2^-31 when clamped
+        if (fabs(x) >= ldexp(1.0, -125))
+            CHECK(fabs(y - x) <= ldexp(fabs(x), clamped ? -31 : -32), "x %.17g y %.17g", x, y);
+        // idempotent: a stored value re-packs to the same code
+        float hi2;
+        int lo2;
+        pk_pack(y, &hi2, &lo2);
+        CHECK(hi2 == hi && lo2 == lo, "x %.17g: (%a, %d) -> (%a, %d)", x, hi, lo, hi2, lo2);
+    }
+    // just under the midpoint between 1 and its FP32 successor: the word is 1,
+    // the correction rounds to 128 and is clamped to 127, an error of 2^-31
+    const double x = 1.0 + ldexp(1.0, -24) - ldexp(1.0, -60);
+    pk_pack(x, &hi, &lo);
+    CHECK(hi == 1.f && lo == 127, "clamp x %.17g: (%a, %d)", x, hi, lo);
+    CHECK(fabs(pk_unpack(hi, lo) - x) <= ldexp(fabs(x), -31), "clamp x %.17g", x);
+}
+
+int main() {
+    for (int natom = 1; natom <= kMaxAtom; ++natom) {
+        check_layout(natom, false);
+        check_layout(natom, true);
+    }
+    check_values();
+    printf("ok\n");
+    return 0;
+}

@@ -265,7 +265,7 @@ def test_ragged_tiles_vs_oracle(shape):
 def test_compensated_state():
     """Lattice r, v of the FP32 form are stored as an FP32 word plus an 8-bit
     compensation term: any double written comes back within 2^-31 of itself
-    (2^-33 but for the clamped last code), values below the FP32 range within
+    (2^-32 but for the clamped last code), values below the FP32 range within
     2^-157, and storing what was read changes nothing."""
     c = golden_cases.load("moo3")
     spec = c["spec"]

@@ -1,0 +1,33 @@
+"""Lattice-state layout (IonLayout) and compensated FP32 value format
+(pk_pack / pk_unpack) of php_fdtd_amd/csrc/phpfdtd_internal.h, checked on the
+host by tests/layout_check.cpp, which includes the header the kernels and the
+state I/O use:
+
+  * for every atom count 1..32 and both state forms, each dof of each slot of
+    two tiles has its own element inside its tile, a lane's group of dofs is
+    one 16-byte vector and the 64 lanes' vectors one 1 KiB row;
+  * compensated form: each dof has its own compensation byte, in the rows
+    after the word rows of its tile, four groups per 16-byte lane vector; the
+    tile is 1 KiB per word row plus 1 KiB per four of them;
+  * r_decode inverts r_index and rejects padding dofs, compensation rows and
+    slots past the end;
+  * pk: zero is exact; the word is fl32(x); a value comes back within half a
+    unit of 2^-8 ulp (2^-32 relative), one unit (2^-31) where the correction
+    is clamped to +-127; re-packing a stored value is the identity.
+"""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_layout_and_value_format(tmp_path):
+    exe = str(tmp_path / "layout_check")
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "--offload-arch=gfx950", os.path.join(HERE, "layout_check.cpp"), "-o",
+                    exe], check=True, capture_output=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout + out.stderr
