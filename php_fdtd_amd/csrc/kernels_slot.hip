@@ -261,6 +261,9 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     constexpr int W = PK ? 4 : 2;  // dofs per lane vector (IonLayout w)
     constexpr int NVX = (N3X + W - 1) / W;
     constexpr int NBX = (NVX + 3) / 4;  // compensation rows (compensated state)
+    // compensated state up to 16 atoms: v is loaded with r, ahead of the MFMA
+    // chain (register room; the runtime-N kernel loads it late)
+    constexpr bool VPRE = PK && NVX <= 12;
     const int natom = exact ? NA : a.natom;
     const int N3 = 3 * natom;
     const int NK = exact ? NKX : (N3 + 3) / 4, NMA = exact ? NAX : (N3 + 15) / 16;
@@ -287,6 +290,8 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     double r[PK ? 1 : W * NVX];
     float rw[PK ? W * NVX : 1];
     u32x4 rc[PK ? NBX : 1];
+    float vw[VPRE ? W * NVX : 1];
+    u32x4 vcr[VPRE ? NBX : 1];
     if constexpr (PK) {
 #pragma unroll
         for (int b = 0; b < NBX; ++b)
@@ -299,6 +304,12 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                 const uint32_t *rb = static_cast<const uint32_t *>(rr_) + tw;
                 const f32x4 h4 = ldquad(rb, lane, j);
                 for (int h = 0; h < 4; ++h) rw[4 * j + h] = h4[h];
+                if constexpr (VPRE) {
+                    const uint32_t *vb = static_cast<const uint32_t *>(vv_) + tw;
+                    const f32x4 g4 = ldquad(vb, lane, j);
+                    for (int h = 0; h < 4; ++h) vw[4 * j + h] = g4[h];
+                    if (j % 4 == 0) vcr[j / 4] = ldcomp(vb, lane, NV, j / 4);
+                }
             } else {
                 const f64x2 v2 = ldpair(static_cast<const double *>(rr_) + tb, j);
                 r[2 * j] = v2.x;
@@ -396,8 +407,14 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
         if (j >= NV) break;
         if constexpr (PK) {
             uint32_t *rb = static_cast<uint32_t *>(rr_) + tw, *vb = static_cast<uint32_t *>(vv_) + tw;
-            if (j % 4 == 0) vcb = ldcomp(vb, lane, NV, j / 4);
-            f32x4 v4 = ldquad(vb, lane, j), r4;
+            f32x4 v4, r4;
+            if constexpr (VPRE) {
+                if (j % 4 == 0) vcb = vcr[j / 4];
+                v4 = (f32x4){vw[4 * j], vw[4 * j + 1], vw[4 * j + 2], vw[4 * j + 3]};
+            } else {
+                if (j % 4 == 0) vcb = ldcomp(vb, lane, NV, j / 4);
+                v4 = ldquad(vb, lane, j);
+            }
             uint32_t vc = vcb[j % 4], rcn = rc[j / 4][j % 4];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
