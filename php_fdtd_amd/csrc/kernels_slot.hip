@@ -15,7 +15,8 @@
 // moves a group of its dofs as one 16-byte vector, so every wavefront access
 // is 1 KiB contiguous, streamed non-temporally: FP64 values in dof pairs, or
 // compensated FP32 (an FP32 word and an 8-bit correction per value) in dof
-// quads with the correction bytes after each 1 KiB group.
+// quads, the tile's correction bytes in 1 KiB rows after its word rows (four
+// groups' bytes per 16-byte lane vector).
 //
 // Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
 // DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
