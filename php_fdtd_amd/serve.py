@@ -25,6 +25,7 @@ same per-tile code on the member's own arguments.
 """
 import collections
 import concurrent.futures as cf
+import os
 import sys
 import threading
 import time
@@ -58,6 +59,15 @@ class Request:
         return None if self.t_done is None else self.t_done - self.t_arrival
 
 
+def default_workers():
+    """Build threads: the host cores this process may use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 4
+    return max(2, min(16, n))
+
+
 def _build(req, precision):
     sim = Simulation(req.spec, req.medium, precision)
     m = req.spec.get("monitor")
@@ -68,13 +78,15 @@ def _build(req, precision):
 
 class Server:
     def __init__(self, precision=_abi.FP32, hbm_budget=0.85 * HBM_BYTES, max_members=64, max_chunk=256,
-                 workers=4, progress=None):
+                 workers=None, progress=None):
         self.precision = precision
         self.progress = progress  # seconds between status lines on stderr (None: quiet)
         self.budget = float(hbm_budget)
         self.max_members = int(max_members)
         self.max_chunk = int(max_chunk)
-        self.workers = int(workers)
+        # context builds (host precompute, uploads) run in parallel: with short
+        # requests they, not the GPU, bound the batch size
+        self.workers = int(workers) if workers else default_workers()
         self.lib = _abi.load_library()
         self._lock = threading.Lock()
         self._in_use = 0.0
