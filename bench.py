@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--serving-step-scale", type=float, default=0.05)
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false",
                     help="skip the measured copy-bandwidth ceilings")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving block and print it as the JSON line (for profiling the batched path)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; the domain is resident on a single MI355X and no "
@@ -340,6 +342,9 @@ def main():
 
     from php_fdtd_amd import _abi, workloads
 
+    if args.serving_only:
+        print(json.dumps({"serving": serving(args.serving, args.serving_step_scale)}))
+        return 0
     prof_steps = 3
     t0 = time.time()
     spec, med = workloads.WORKLOADS[args.workload]()
