@@ -84,9 +84,10 @@ struct Special {
     SlotArgs<float> sa32{};
 };
 
-// Stages of one time step, in issue order.  With overlap, the H ranges that
-// read no lattice-cell E (K_HA) of step n+1 run on a second stream beside the
-// lattice kernels (K_ION) of step n; every other order is sequential.
+// Stages of one time step, in issue order.  With overlap (PHPFDTD_OVERLAP=1),
+// the H ranges that read no lattice-cell E (K_HA) of step n+1 run on a second
+// stream beside the lattice kernels (K_ION) of step n; every other order, and
+// the default plan, is sequential.
 enum Stage { K_HA, K_HB, K_DE, K_PRE, K_ION, K_POST };
 
 struct Kernel {
@@ -160,7 +161,7 @@ struct phpfdtd_ctx {
     hipEvent_t ev = nullptr;  // marks this stream's latest work for the batch stream
     // plan and graphs
     std::vector<Kernel> plan;
-    bool allow_overlap = true;            // PHPFDTD_NO_OVERLAP=1 turns the overlap off
+    bool allow_overlap = false;           // PHPFDTD_OVERLAP=1 turns the overlap on
     bool overlap = false;                 // plan has K_HA ranges to overlap with K_ION
     hipStream_t stream2 = nullptr;        // second stream of the overlapped graphs
     std::vector<hipEvent_t> cap_events;   // fork/join events of the captured graphs
@@ -630,8 +631,12 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     x->nmat = nmat;
     const char *ng = getenv("PHPFDTD_NO_GRAPH");
     x->use_graph = !(ng && ng[0] == '1');
-    const char *no = getenv("PHPFDTD_NO_OVERLAP");
-    x->allow_overlap = !(no && no[0] == '1');
+    // H ranges beside the lattice kernel: off by default.  Every kernel of the
+    // step streams near the HBM limit, so running two side by side gains
+    // nothing, while splitting H into three launches costs 0.3 ms (FP32 step,
+    // 1024x1024x640: 21.69 ms with one H launch, 22.08 ms overlapped)
+    const char *ov = getenv("PHPFDTD_OVERLAP");
+    x->allow_overlap = ov && ov[0] == '1';
     // Yee tile shape, defaults measured on MI355X (1024x1024x640, same-context
     // comparisons): FP32 1 y row x 4 planes, FP64 2 y rows x 16 planes;
     // overridable for tuning

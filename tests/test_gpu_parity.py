@@ -304,11 +304,14 @@ def test_compensated_vs_fp64_state(name):
     assert abs(a[2].max_r - b[2].max_r) <= 1e-5 * max(b[2].max_r, 1e-300)
 
 
+@pytest.mark.parametrize("overlap", ["0", "1"])
 @pytest.mark.parametrize("case", ["hbn", "tip_slab/8"])
-def test_graph_matches_stream_launches(monkeypatch, case):
-    """16-step graphs (with the next step's lattice-free H ranges on a second
-    stream beside the lattice kernel) give the same bits as plain sequential
-    launches, with and without an ADE material beside the lattice."""
+def test_graph_matches_stream_launches(monkeypatch, case, overlap):
+    """16-step graphs give the same bits as plain sequential launches, with
+    and without an ADE material beside the lattice: the default plan, and the
+    overlapped one (PHPFDTD_OVERLAP=1: the next step's lattice-free H ranges
+    on a second stream beside the lattice kernel)."""
+    monkeypatch.setenv("PHPFDTD_OVERLAP", overlap)
     if case == "hbn":
         c = golden_cases.load("hbn")
         spec, med, precs = c["spec"], c["medium"], (_abi.FP64,)
