@@ -230,13 +230,16 @@ def ragged_case(shape):
     return spec, med
 
 
+@pytest.mark.parametrize("overlap", ["0", "1"])
 @pytest.mark.parametrize("shape", [(300, 13, 21), (257, 9, 18)])
-def test_ragged_tiles_vs_oracle(shape):
+def test_ragged_tiles_vs_oracle(monkeypatch, shape, overlap):
     """Every field, lattice and ADE state of both forms against the oracle
     from a random state on grids with ragged x tiles, y blocks and z chunks;
-    the lattice slab also splits the H update into three z ranges."""
+    with PHPFDTD_OVERLAP=1 the lattice slab also splits the H update into
+    three z ranges (chunks counted from each range's first plane)."""
+    monkeypatch.setenv("PHPFDTD_OVERLAP", overlap)
     spec, med = ragged_case(shape)
-    print("ragged case", shape, flush=True)
+    print("ragged case", shape, "overlap", overlap, flush=True)
     o = orc.Oracle(spec, med)
     for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL)):
         sim = Simulation(spec, med, prec)
