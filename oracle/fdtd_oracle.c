@@ -10,16 +10,16 @@
  * single-rank behaviour of fdtd.c (reference file, cited by line):
  *   init_pml                      fdtd.c:962-989
  *   inverse / determinant         fdtd.c:205-258
- *   inverse_dielectric_tensor     fdtd.c:555-625   (pseudo-complex, Q7)
+ *   inverse_dielectric_tensor     fdtd.c:555-625   (pseudo-complex)
  *   built-in materials            fdtd.c:628-715
  *   ADE coefficients              fdtd.c:2478-2528
  *   calculate_source_value        fdtd.c:1923-1947
- *   H update                      fdtd.c:3123-3231 (lagged psi, Q1)
- *   D update + eps_im loss        fdtd.c:3238-3356 (Q5)
+ *   H update                      fdtd.c:3123-3231 (lagged psi)
+ *   D update + eps_im loss        fdtd.c:3238-3356 (unrotated eps_im)
  *   source injection              fdtd.c:3363-3396
  *   D->E, ADE, ion EOM            fdtd.c:3404-3531
  *   diagnostics                   fdtd.c:3535-3596
- *   monitor sampling (z+1, Q2)    fdtd.c:3806-3851
+ *   monitor sampling (z+1)        fdtd.c:3806-3851
  * with one deliberate difference: the ADE update uses the intended full 3x3
  * recurrence (all of P^{n-1} and of the right-hand side before the solve).
  * The reference's loop (fdtd.c:3465-3475) reads uninitialised tempvec entries
@@ -721,7 +721,7 @@ void orc_diag(const orc_sim *s, double out[3], int idx[5]) {
 }
 
 /* float32 slice in the reference layout (fdtd.c:1051-1059, 1121-1126,
- * 1314-1319); the z slice holds plane pos+1 (Q3).  comp 9 = |E|. */
+ * 1314-1319); the z slice holds plane pos+1.  comp 9 = |E|. */
 int orc_slice(const orc_sim *s, int comp, char axis, int pos, float *out) {
     int nx = s->nx, ny = s->ny, nz = s->nz;
 #define VAL(i, j, k)                                                                              \

@@ -98,7 +98,7 @@ AxisHost axis_tables(int n, double delta, const phpfdtd_grid &g) {
             slot = i;
         } else if (L > 0 && i >= n - L) {
             p = n - 1 - i;
-            slot = L + p;
+            slot = L + (i - (n - L));  // ascending in i, so a lane's x vector is contiguous in its slab
         }
         double kappa = 1.0, b = 0.0, c = 0.0;
         if (p >= 0) {

@@ -220,7 +220,7 @@ void parse_material(Cfg &c, const std::string &line) {
         } else if (k == "anglez") m.anglez = atof(v.c_str());
         else if (k == "anglex") m.anglex = atof(v.c_str());
         else if (k == "born") {
-            // a file name; the inline [..] form cannot survive the comma split (Q9)
+            // a file name; the inline [..] form cannot survive the comma split
             if (v.find('[') == std::string::npos && v.find(']') == std::string::npos) {
                 m.born_file = v;
                 m.born_eps = true;
@@ -464,7 +464,7 @@ void load_materials(Cfg &c) {
             printf("read BORN charges\n");
             double re[9];
             if (!read_outcar_eps(m.born_file, n, re)) die("Failed to read Born charges from %s", m.born_file.c_str());
-            if (m.born_eps) {  // born= overwrites eps and zeroes eps_im (Q9)
+            if (m.born_eps) {  // born= overwrites eps and zeroes eps_im
                 memcpy(m.re, re, sizeof(re));
                 memset(m.im, 0, sizeof(m.im));
                 if (m.switchxz == 1) {
@@ -608,7 +608,7 @@ int main(int argc, char **argv) {
     const char *fmt = precise ? "%.17e" : "%e";
     FILE *mon_f = nullptr, *full_f = nullptr;
     if (mon_in) {
-        mon_f = fopen(c.mon_file.c_str(), "w");  // an empty name opens nothing (Q15)
+        mon_f = fopen(c.mon_file.c_str(), "w");  // an empty name opens nothing
         if (mon_f)
             fprintf(mon_f, mon_rv ? "# Time(s)\tEx\tEy\tEz\tDx\tDy\tDz\tHx\tHy\tHz\tR\tV\n"
                                   : "# Time(s)\tEx\tEy\tEz\tDx\tDy\tDz\tHx\tHy\tHz\n");

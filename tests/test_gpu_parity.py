@@ -206,17 +206,19 @@ def test_lattice_atom_counts(natom):
         s.close()
 
 
-def ragged_case(shape):
+def ragged_case(shape, L=3):
     """A grid whose x, y and z extents end inside a wavefront tile, a y block
     and a z chunk of both forms (FP32: 256-cell x tiles, 4-plane chunks;
     FP64: 128-cell x tiles, 2-row waves, 16-plane chunks), with a 3-cell CPML
     on every face, a rotated 4-atom lattice slab and a Drude ADE block that
     straddle the x tile seams, a lossy dielectric block, and Gaussian sources
     (peak at t = 0, so they inject in the first steps) on both sides of each
-    seam."""
+    seam.  With L = 3 the x-PML lanes mix PML and interior cells (per-cell
+    psi path); with L = 4 and nx a multiple of 4 every x-PML lane holds four
+    consecutive slab cells (vector psi path)."""
     nx, ny, nz = shape
     f = 1000.0 * workloads.WN_TO_HZ
-    g = workloads._grid(shape, 20e-9, 40, 3, f)
+    g = workloads._grid(shape, 20e-9, 40, L, f)
     lossy = dict(workloads.material_lorentz(f), use_ade=0)
     mats = workloads.builtin_table(f) + [workloads.material_lattice4(), workloads.material_drude_tip(f), lossy]
     med = np.zeros(shape, np.int32)
@@ -231,15 +233,15 @@ def ragged_case(shape):
 
 
 @pytest.mark.parametrize("overlap", ["0", "1"])
-@pytest.mark.parametrize("shape", [(300, 13, 21), (257, 9, 18)])
-def test_ragged_tiles_vs_oracle(monkeypatch, shape, overlap):
+@pytest.mark.parametrize("shape,L", [((300, 13, 21), 3), ((257, 9, 18), 3), ((264, 12, 20), 4)])
+def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, overlap):
     """Every field, lattice and ADE state of both forms against the oracle
     from a random state on grids with ragged x tiles, y blocks and z chunks;
     with PHPFDTD_OVERLAP=1 the lattice slab also splits the H update into
     three z ranges (chunks counted from each range's first plane)."""
     monkeypatch.setenv("PHPFDTD_OVERLAP", overlap)
-    spec, med = ragged_case(shape)
-    print("ragged case", shape, "overlap", overlap, flush=True)
+    spec, med = ragged_case(shape, L)
+    print("ragged case", shape, "L", L, "overlap", overlap, flush=True)
     o = orc.Oracle(spec, med)
     for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL)):
         sim = Simulation(spec, med, prec)
