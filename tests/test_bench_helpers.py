@@ -1,0 +1,42 @@
+"""bench.py's link to the committed profile: every step kernel of the plan
+maps to a device symbol that has rows in the kernel trace and the PMC traffic
+table of profiles/step_kernels_tip_slab_1024x1024x640.md, and the traffic it
+reads back for the lattice kernel equals that kernel's compulsory bytes (the
+PMC table's traffic / algorithmic ratio is 1.000 for it)."""
+import os
+import re
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+PLAN = ["h_update", "de_update", "ade_update[m5]", "ion_update[m4,N16]", "step_end"]
+
+
+def profile_symbols(section):
+    text = open(bench.PROFILE).read()
+    part = text.split(section, 1)[1].split("\n## ", 1)[0]
+    return set(re.findall(r"^\| `([^`]+)` \|", part, re.M))
+
+
+def test_plan_kernels_have_profile_rows():
+    trace = profile_symbols("## Kernel trace")
+    pmc = profile_symbols("## HBM traffic")
+    for form in ("fp32", "fp64"):
+        for name in PLAN:
+            sym = bench.kernel_symbol(name, form)
+            assert sym in trace, (form, name, sym)
+            assert sym in pmc, (form, name, sym)
+
+
+def test_lattice_kernel_traffic_is_its_compulsory_bytes():
+    # 50,331,648 lattice slots x (6w + 4 + 12 N b) bytes, N = 16, w = 4, b = 5 (compensated)
+    slots = 1024 * 1024 * 48
+    algorithmic = slots * (6 * 4 + 4 + 12 * 16 * 5)
+    t = bench.profile_traffic(bench.kernel_symbol("ion_update[m4,N16]", "fp32"))
+    assert t is not None and abs(t / algorithmic - 1) < 0.002, (t, algorithmic)
+    t64 = bench.profile_traffic(bench.kernel_symbol("ion_update[m4,N16]", "fp64"))
+    algorithmic64 = slots * (6 * 8 + 4 + 12 * 16 * 8)
+    assert t64 is not None and abs(t64 / algorithmic64 - 1) < 0.002, (t64, algorithmic64)
