@@ -54,6 +54,7 @@ extern "C" {
 #define PHPFDTD_EDIVERGED  -4  /* max|r| > 1e-10 m at a sync (fdtd.c:3626)      */
 #define PHPFDTD_EFORMAT    -5  /* malformed input data                          */
 #define PHPFDTD_ESTATE     -6  /* call not valid in the context's current state */
+#define PHPFDTD_ENOMEM_HOST -7 /* host allocation failed                        */
 
 /* Numerical forms. */
 #define PHPFDTD_FP64 0  /* plain form: everything in FP64                          */

@@ -20,6 +20,7 @@ EHIP = -3
 EDIVERGED = -4
 EFORMAT = -5
 ESTATE = -6
+ENOMEM_HOST = -7
 
 FP64 = 0
 FP32 = 1       # precision form, lattice r, v stored as compensated FP32

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <functional>
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -588,6 +590,22 @@ static int upload_mats(phpfdtd_ctx *x) {
     return x->upload(&x->d_mats, mc);
 }
 
+// Every int entry point runs its body through guarded(): a C++ exception
+// (a host allocation failure, say) becomes an error code instead of crossing
+// the C ABI and ending the caller's process.
+template <typename F>
+static int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return fail(PHPFDTD_ENOMEM_HOST, "host memory allocation failed");
+    } catch (const std::exception &e) {
+        return fail(PHPFDTD_EINVAL, "internal error: %s", e.what());
+    } catch (...) {
+        return fail(PHPFDTD_EINVAL, "internal error");
+    }
+}
+
 extern "C" {
 
 int phpfdtd_abi_version(void) { return PHPFDTD_ABI_VERSION; }
@@ -596,7 +614,7 @@ double phpfdtd_source_value(const phpfdtd_source *src, double t) { return src ? 
 
 const char *phpfdtd_last_error(void) { return g_err.c_str(); }
 
-int phpfdtd_footprint_bytes(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
+static int footprint_bytes_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
                             size_t *bytes) {
     int rc = check_inputs(g, mats, nmat, medium);
     if (rc) return rc;
@@ -612,7 +630,12 @@ int phpfdtd_footprint_bytes(const phpfdtd_grid *g, const phpfdtd_material *mats,
     return 0;
 }
 
-int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
+int phpfdtd_footprint_bytes(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
+                            size_t *bytes) {
+    return guarded([&] { return footprint_bytes_impl(g, mats, nmat, medium, bytes); });
+}
+
+static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
                    phpfdtd_ctx **out) {
     if (!out) return fail(PHPFDTD_EINVAL, "null out");
     *out = nullptr;
@@ -775,6 +798,11 @@ int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat
     return 0;
 }
 
+int phpfdtd_create(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
+                   phpfdtd_ctx **out) {
+    return guarded([&] { return create_impl(g, mats, nmat, medium, out); });
+}
+
 void phpfdtd_destroy(phpfdtd_ctx *x) {
     if (!x) return;
     if (x->batch) phpfdtd_batch_remove(x->batch, x);
@@ -783,7 +811,7 @@ void phpfdtd_destroy(phpfdtd_ctx *x) {
 
 int64_t phpfdtd_step(const phpfdtd_ctx *x) { return x ? x->host_step : -1; }
 
-int phpfdtd_set_sources(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
+static int set_sources_impl(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
     if (!x || (n > 0 && !src) || n < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
     // keep the sources that can ever inject (fdtd.c:3375-3393 skips the rest)
@@ -823,6 +851,10 @@ int phpfdtd_set_sources(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
     return 0;
 }
 
+int phpfdtd_set_sources(phpfdtd_ctx *x, int n, const phpfdtd_source *src) {
+    return guarded([&] { return set_sources_impl(x, n, src); });
+}
+
 static int find_slot(const phpfdtd_ctx *x, int sp, size_t c, int64_t *slot) {
     const auto &v = x->specials[sp].cells;
     auto it = std::lower_bound(v.begin(), v.end(), (uint32_t)c);
@@ -831,7 +863,7 @@ static int find_slot(const phpfdtd_ctx *x, int sp, size_t c, int64_t *slot) {
     return 0;
 }
 
-int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int dir, int ref_compat) {
+static int set_monitor_impl(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int dir, int ref_compat) {
     if (!x) return fail(PHPFDTD_EINVAL, "null ctx");
     if (mx < 0 || mx >= x->nx || my < 0 || my >= x->ny || mz < 0 || mz >= x->nz || dir < 0 || dir > 2 || atom < 0)
         return fail(PHPFDTD_EINVAL, "monitor point (%d,%d,%d) atom %d dir %d out of range", mx, my, mz, atom, dir);
@@ -859,6 +891,10 @@ int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int di
     return 0;
 }
 
+int phpfdtd_set_monitor(phpfdtd_ctx *x, int mx, int my, int mz, int atom, int dir, int ref_compat) {
+    return guarded([&] { return set_monitor_impl(x, mx, my, mz, atom, dir, ref_compat); });
+}
+
 // Plane of phpfdtd_read_slice / phpfdtd_set_dft: element count and the plane
 // index actually read (ref_compat z reads pos+1, which may be the zero halo nz).
 static int plane_of(const phpfdtd_ctx *x, char axis, int pos, int ref_compat, size_t *n, int *p) {
@@ -881,7 +917,7 @@ static int plane_of(const phpfdtd_ctx *x, char axis, int pos, int ref_compat, si
     return 0;
 }
 
-int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
+static int set_dft_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
                     double f_hz, int *bin) {
     if (!x || comp < 0 || comp > 9 || interval < 1 || nsamples < 2)
         return fail(PHPFDTD_EINVAL, "bad arguments (comp %d, interval %d, nsamples %d)", comp, interval, nsamples);
@@ -946,7 +982,12 @@ int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat
     return 0;
 }
 
-int phpfdtd_read_dft(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
+int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
+                    double f_hz, int *bin) {
+    return guarded([&] { return set_dft_impl(x, comp, axis, pos, ref_compat, interval, nsamples, f_hz, bin); });
+}
+
+static int read_dft_impl(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
     if (!x) return fail(PHPFDTD_EINVAL, "null ctx");
     if (!x->dft_on) return fail(PHPFDTD_ESTATE, "no DFT set (phpfdtd_set_dft)");
     HIPCHK(hipSetDevice(x->device));
@@ -967,7 +1008,11 @@ int phpfdtd_read_dft(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
     return 0;
 }
 
-int phpfdtd_run(phpfdtd_ctx *x, int nsteps) {
+int phpfdtd_read_dft(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
+    return guarded([&] { return read_dft_impl(x, re, im, n_samples); });
+}
+
+static int run_impl(phpfdtd_ctx *x, int nsteps) {
     if (!x || nsteps < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
     if (x->batch) return fail(PHPFDTD_ESTATE, "context is a batch member; step it with phpfdtd_batch_run");
     if (x->host_step + nsteps > x->g.tsteps)
@@ -993,7 +1038,11 @@ int phpfdtd_run(phpfdtd_ctx *x, int nsteps) {
     return 0;
 }
 
-int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
+int phpfdtd_run(phpfdtd_ctx *x, int nsteps) {
+    return guarded([&] { return run_impl(x, nsteps); });
+}
+
+static int sync_impl(phpfdtd_ctx *x, phpfdtd_diag *d) {
     if (!x) return fail(PHPFDTD_EINVAL, "null ctx");
     HIPCHK(hipSetDevice(x->device));
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1038,7 +1087,11 @@ int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
     return 0;
 }
 
-int phpfdtd_read_slice(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, float *out) {
+int phpfdtd_sync(phpfdtd_ctx *x, phpfdtd_diag *d) {
+    return guarded([&] { return sync_impl(x, d); });
+}
+
+static int read_slice_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, float *out) {
     if (!x || !out || comp < 0 || comp > 9) return fail(PHPFDTD_EINVAL, "bad arguments");
     size_t n;
     int p;
@@ -1064,7 +1117,11 @@ int phpfdtd_read_slice(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_com
     return 0;
 }
 
-int phpfdtd_read_monitor(phpfdtd_ctx *x, double *rows, int max_rows, int *n_rows) {
+int phpfdtd_read_slice(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, float *out) {
+    return guarded([&] { return read_slice_impl(x, comp, axis, pos, ref_compat, out); });
+}
+
+static int read_monitor_impl(phpfdtd_ctx *x, double *rows, int max_rows, int *n_rows) {
     if (!x || !n_rows || max_rows < 0 || (max_rows > 0 && !rows)) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
     *n_rows = 0;
@@ -1079,7 +1136,11 @@ int phpfdtd_read_monitor(phpfdtd_ctx *x, double *rows, int max_rows, int *n_rows
     return 0;
 }
 
-int phpfdtd_read_point(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
+int phpfdtd_read_monitor(phpfdtd_ctx *x, double *rows, int max_rows, int *n_rows) {
+    return guarded([&] { return read_monitor_impl(x, rows, max_rows, n_rows); });
+}
+
+static int read_point_impl(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
     if (!x || !f9 || px < 0 || px >= x->nx || py < 0 || py >= x->ny || pz < 0 || pz > x->nz)
         return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
@@ -1099,7 +1160,11 @@ int phpfdtd_read_point(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
     return 0;
 }
 
-int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double *v) {
+int phpfdtd_read_point(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
+    return guarded([&] { return read_point_impl(x, px, py, pz, f9); });
+}
+
+static int read_ions_impl(phpfdtd_ctx *x, int px, int py, int pz, double *r, double *v) {
     if (!x || px < 0 || px >= x->nx || py < 0 || py >= x->ny || pz < 0 || pz >= x->nz)
         return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
@@ -1134,7 +1199,11 @@ int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double 
     return x->pre[m].natom;
 }
 
-int phpfdtd_read_field(phpfdtd_ctx *x, int comp, double *out) {
+int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double *v) {
+    return guarded([&] { return read_ions_impl(x, px, py, pz, r, v); });
+}
+
+static int read_field_impl(phpfdtd_ctx *x, int comp, double *out) {
     if (!x || !out || comp < 0 || comp > 8) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1150,7 +1219,11 @@ int phpfdtd_read_field(phpfdtd_ctx *x, int comp, double *out) {
     return 0;
 }
 
-int phpfdtd_write_field(phpfdtd_ctx *x, int comp, const double *in) {
+int phpfdtd_read_field(phpfdtd_ctx *x, int comp, double *out) {
+    return guarded([&] { return read_field_impl(x, comp, out); });
+}
+
+static int write_field_impl(phpfdtd_ctx *x, int comp, const double *in) {
     if (!x || !in || comp < 0 || comp > 8) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1167,11 +1240,19 @@ int phpfdtd_write_field(phpfdtd_ctx *x, int comp, const double *in) {
     return 0;
 }
 
-int phpfdtd_count_cells(phpfdtd_ctx *x, int mat, int64_t *n) {
+int phpfdtd_write_field(phpfdtd_ctx *x, int comp, const double *in) {
+    return guarded([&] { return write_field_impl(x, comp, in); });
+}
+
+static int count_cells_impl(phpfdtd_ctx *x, int mat, int64_t *n) {
     if (!x || !n || mat < 0 || mat >= x->nmat) return fail(PHPFDTD_EINVAL, "bad arguments");
     const int sp = x->special_of[mat];
     *n = sp >= 0 ? (int64_t)x->specials[sp].cells.size() : 0;
     return 0;
+}
+
+int phpfdtd_count_cells(phpfdtd_ctx *x, int mat, int64_t *n) {
+    return guarded([&] { return count_cells_impl(x, mat, n); });
 }
 
 // order of a material's slots in file order (i slowest, k fastest)
@@ -1279,13 +1360,19 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     return 0;
 }
 
-int phpfdtd_read_state(phpfdtd_ctx *x, int mat, int kind, double *out) { return state_io(x, mat, kind, out, 0); }
+int phpfdtd_read_state(phpfdtd_ctx *x, int mat, int kind, double *out) {
+    return guarded([&] { return state_io(x, mat, kind, out, 0); });
+}
 
-int phpfdtd_write_state(phpfdtd_ctx *x, int mat, int kind, const double *in) {
+static int write_state_impl(phpfdtd_ctx *x, int mat, int kind, const double *in) {
     return state_io(x, mat, kind, const_cast<double *>(in), 1);
 }
 
-int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out, int *n_out) {
+int phpfdtd_write_state(phpfdtd_ctx *x, int mat, int kind, const double *in) {
+    return guarded([&] { return write_state_impl(x, mat, kind, in); });
+}
+
+static int profile_impl(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out, int *n_out) {
     if (!x || nsteps <= 0 || !n_out) return fail(PHPFDTD_EINVAL, "bad arguments");
     if (x->host_step + nsteps > x->g.tsteps) return fail(PHPFDTD_EINVAL, "profile past tsteps");
     if (x->batch) return fail(PHPFDTD_ESTATE, "context is a batch member");
@@ -1326,7 +1413,11 @@ int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out,
     return 0;
 }
 
-int phpfdtd_time_steps(phpfdtd_ctx *x, int nsteps, double *ms) {
+int phpfdtd_profile(phpfdtd_ctx *x, int nsteps, phpfdtd_ktime *out, int max_out, int *n_out) {
+    return guarded([&] { return profile_impl(x, nsteps, out, max_out, n_out); });
+}
+
+static int time_steps_impl(phpfdtd_ctx *x, int nsteps, double *ms) {
     if (!x || !ms || nsteps <= 0) return fail(PHPFDTD_EINVAL, "bad arguments");
     if (x->batch) return fail(PHPFDTD_ESTATE, "context is a batch member");
     HIPCHK(hipSetDevice(x->device));
@@ -1351,11 +1442,19 @@ int phpfdtd_time_steps(phpfdtd_ctx *x, int nsteps, double *ms) {
     return 0;
 }
 
-int phpfdtd_struct_sizes(int *sizes, int n) {
+int phpfdtd_time_steps(phpfdtd_ctx *x, int nsteps, double *ms) {
+    return guarded([&] { return time_steps_impl(x, nsteps, ms); });
+}
+
+static int struct_sizes_impl(int *sizes, int n) {
     const int s[5] = {(int)sizeof(phpfdtd_grid), (int)sizeof(phpfdtd_material), (int)sizeof(phpfdtd_source),
                       (int)sizeof(phpfdtd_diag), (int)sizeof(phpfdtd_ktime)};
     for (int q = 0; q < n && q < 5; ++q) sizes[q] = s[q];
     return 5;
+}
+
+int phpfdtd_struct_sizes(int *sizes, int n) {
+    return guarded([&] { return struct_sizes_impl(sizes, n); });
 }
 
 }  // extern "C"
@@ -1554,7 +1653,7 @@ static void batch_step_t(phpfdtd_batch *b) {
 
 extern "C" {
 
-int phpfdtd_batch_create(int precision, phpfdtd_batch **out) {
+static int batch_create_impl(int precision, phpfdtd_batch **out) {
     if (!out || (precision != PHPFDTD_FP64 && precision != PHPFDTD_FP32 && precision != PHPFDTD_FP32_RV64))
         return fail(PHPFDTD_EINVAL, "bad arguments");
     *out = nullptr;
@@ -1572,7 +1671,11 @@ int phpfdtd_batch_create(int precision, phpfdtd_batch **out) {
     return 0;
 }
 
-int phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *x) {
+int phpfdtd_batch_create(int precision, phpfdtd_batch **out) {
+    return guarded([&] { return batch_create_impl(precision, out); });
+}
+
+static int batch_add_impl(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (!b || !x) return fail(PHPFDTD_EINVAL, "null argument");
     if (x->batch) return fail(PHPFDTD_ESTATE, "context is already in a batch");
     if (x->g.precision != b->form)
@@ -1590,7 +1693,11 @@ int phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *x) {
     return 0;
 }
 
-int phpfdtd_batch_remove(phpfdtd_batch *b, phpfdtd_ctx *x) {
+int phpfdtd_batch_add(phpfdtd_batch *b, phpfdtd_ctx *x) {
+    return guarded([&] { return batch_add_impl(b, x); });
+}
+
+static int batch_remove_impl(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (!b || !x) return fail(PHPFDTD_EINVAL, "null argument");
     auto it = std::find(b->members.begin(), b->members.end(), x);
     if (it == b->members.end()) return fail(PHPFDTD_EINVAL, "context is not a member of this batch");
@@ -1601,7 +1708,11 @@ int phpfdtd_batch_remove(phpfdtd_batch *b, phpfdtd_ctx *x) {
     return 0;
 }
 
-int phpfdtd_batch_run(phpfdtd_batch *b, int nsteps) {
+int phpfdtd_batch_remove(phpfdtd_batch *b, phpfdtd_ctx *x) {
+    return guarded([&] { return batch_remove_impl(b, x); });
+}
+
+static int batch_run_impl(phpfdtd_batch *b, int nsteps) {
     if (!b || nsteps < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
     for (phpfdtd_ctx *x : b->members)
         if (x->host_step + nsteps > x->g.tsteps)
@@ -1641,7 +1752,11 @@ int phpfdtd_batch_run(phpfdtd_batch *b, int nsteps) {
     return 0;
 }
 
-int phpfdtd_batch_sync(phpfdtd_batch *b) {
+int phpfdtd_batch_run(phpfdtd_batch *b, int nsteps) {
+    return guarded([&] { return batch_run_impl(b, nsteps); });
+}
+
+static int batch_sync_impl(phpfdtd_batch *b) {
     if (!b) return fail(PHPFDTD_EINVAL, "null batch");
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1650,7 +1765,11 @@ int phpfdtd_batch_sync(phpfdtd_batch *b) {
     return 0;
 }
 
-int phpfdtd_batch_wait(phpfdtd_batch *b, int keep) {
+int phpfdtd_batch_sync(phpfdtd_batch *b) {
+    return guarded([&] { return batch_sync_impl(b); });
+}
+
+static int batch_wait_impl(phpfdtd_batch *b, int keep) {
     if (!b || keep < 0) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(b->device));
     while ((int)b->inflight.size() > keep) {
@@ -1659,6 +1778,10 @@ int phpfdtd_batch_wait(phpfdtd_batch *b, int keep) {
         b->inflight.erase(b->inflight.begin());
     }
     return 0;
+}
+
+int phpfdtd_batch_wait(phpfdtd_batch *b, int keep) {
+    return guarded([&] { return batch_wait_impl(b, keep); });
 }
 
 int phpfdtd_batch_size(const phpfdtd_batch *b) { return b ? (int)b->members.size() : 0; }
