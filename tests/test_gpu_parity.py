@@ -235,7 +235,7 @@ def ragged_case(shape, L=3):
 @pytest.mark.parametrize("overlap", ["0", "1"])
 @pytest.mark.parametrize("shape,L", [((300, 13, 21), 3), ((257, 9, 18), 3), ((264, 12, 20), 4)])
 def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, overlap):
-    """Every field, lattice and ADE state of both forms against the oracle
+    """Every field, lattice and ADE state of every form against the oracle
     from a random state on grids with ragged x tiles, y blocks and z chunks;
     with PHPFDTD_OVERLAP=1 the lattice slab also splits the H update into
     three z ranges (chunks counted from each range's first plane)."""
@@ -243,7 +243,7 @@ def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, overlap):
     spec, med = ragged_case(shape, L)
     print("ragged case", shape, "L", L, "overlap", overlap, flush=True)
     o = orc.Oracle(spec, med)
-    for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL)):
+    for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL), (_abi.FP32_RV64, FP32_TOL)):
         sim = Simulation(spec, med, prec)
         oo = orc.Oracle(spec, med) if prec != _abi.FP64 else o
         randomize(spec, med, 7, sim, oo)
