@@ -8,9 +8,11 @@ step (H, D/E, ADE, lattice, monitor epilogue) of the whole domain.
 
 Reported: Gcell-updates/s of the FP32 precision form (value) and of the FP64
 plain form, the HBM roofline of the dominant kernel (HIP events on the
-library's stream, algorithmic bytes), the whole-step byte model, the FP32 vs
-FP64 monitor error over the timed run, and the reference solver on the host
-cores on a bounded sample of the same workload (cpu_baseline).
+library's stream, algorithmic bytes), the whole-step byte model, the time of
+one phpfdtd_sync with diagnostics, the FP32 vs FP64 error on the same
+workload over 5000 steps (monitor trace and near-field DFT at the source
+frequency), the serving mix, and the reference solver on the host cores
+(cpu_baseline: the workload's own grid when host RAM holds it).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tip_slab]
 """
@@ -114,19 +116,24 @@ def run_form(spec, med, precision, warmup, steps, prof_steps):
     # the same steps with output sampling: a z slice and the monitor rows read
     # back every `interval` steps, as a reference run with output_interval does
     interval, chunks = 10, max(1, steps // 10)
+    rows = [sim.monitor()]  # every row since the start (the ring drains on read)
     sim.sync(diag=False)
     t0 = time.perf_counter()
     for _ in range(chunks):
         sim.run(interval)
         sim.slice("Ez", "z", spec["sources"][0]["z"])
-        sim.monitor()
+        rows.append(sim.monitor())
     sim.sync(diag=False)
     sampled_ms = (time.perf_counter() - t0) * 1e3 / (chunks * interval)
-    rows = sim.monitor()
-    d = sim.sync()
+    rows.append(sim.monitor())
+    # phpfdtd_sync with diagnostics (max |E|, max |Ex|, max |r| and its argmax)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        d = sim.sync()
+    sync_ms = (time.perf_counter() - t0) * 1e3 / 3
     sim.close()
-    return dict(ms=ms, wall_ms=wall * 1e3 / steps, prof=prof, rows=rows, max_e=d.max_e, max_r=d.max_r,
-                sampled_ms=sampled_ms)
+    return dict(ms=ms, wall_ms=wall * 1e3 / steps, prof=prof, rows=np.concatenate(rows), max_e=d.max_e,
+                max_r=d.max_r, sampled_ms=sampled_ms, sync_ms=sync_ms)
 
 
 def central(a):
@@ -137,11 +144,12 @@ def central(a):
     return a[(i > n0 * 0.25) & (i < n0 * 0.75) & (j > n1 * 0.25) & (j < n1 * 0.75)]
 
 
-def accuracy(name, interval=10, steps=None):
-    """FP32 precision form vs FP64 plain form over a full-length run: monitor
-    trace relative L2 (fields; lattice R, V) and the near-field single-bin DFT
-    at the source frequency of the z plane one cell above the slab, central
-    25-75% region (frequency_analysis.c semantics)."""
+def accuracy(name, interval=10, steps=None, rv64=True):
+    """FP32 precision form vs FP64 plain form over a run of the workload:
+    monitor trace relative L2 (fields; lattice R, V) and the near-field
+    single-bin DFT at the source frequency of the z plane one cell above the
+    slab (material 4), central 25-75% region (frequency_analysis.c
+    semantics).  rv64 adds the FP32 form with FP64 lattice r, v."""
     from php_fdtd_amd import _abi, workloads
     from php_fdtd_amd.sim import Simulation
 
@@ -151,13 +159,13 @@ def accuracy(name, interval=10, steps=None):
     g = spec["grid"]
     T = g["tsteps"]
     N = T // interval
-    zs = np.nonzero((med != 0).any(axis=(0, 1)))[0]
+    zs = np.nonzero((med == 4).any(axis=(0, 1)))[0]
     zplane = int(zs.max()) + 1
     f = spec["sources"][-1]["frequency"]
     mon = spec["monitor"]
     res = {}
     names = {_abi.FP32: "fp32", _abi.FP32_RV64: "fp32 (r,v fp64)", _abi.FP64: "fp64"}
-    for prec in (_abi.FP32, _abi.FP32_RV64, _abi.FP64):
+    for prec in ((_abi.FP32, _abi.FP32_RV64, _abi.FP64) if rv64 else (_abi.FP32, _abi.FP64)):
         sim = Simulation(spec, med, prec)
         sim.set_monitor(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"], ref_compat=True)
         k = sim.set_dft("Ez", "z", zplane, interval, N, f, ref_compat=False)
@@ -171,7 +179,7 @@ def accuracy(name, interval=10, steps=None):
         sim.close()
         res[prec] = dict(rows=rows, x=x, k=k, wall=wall, max_r=d.max_r)
         log("  accuracy %s %s: %d steps in %.2f s" % (name, names[prec], T, wall))
-    a, b, c = res[_abi.FP32], res[_abi.FP64], res[_abi.FP32_RV64]
+    a, b, c = res[_abi.FP32], res[_abi.FP64], res.get(_abi.FP32_RV64)
     x32, x64 = central(a["x"]), central(b["x"])
     amp32, amp64 = np.abs(x32), np.abs(x64)
     ph = np.angle(x32 * np.conj(x64))
@@ -182,21 +190,32 @@ def accuracy(name, interval=10, steps=None):
         n = np.linalg.norm(v)
         return float(np.linalg.norm(u - v) / n) if n else None
 
-    return {
+    src = spec["sources"][0]
+    out = {
         "workload": "%s%s: %dx%dx%d cells, %d steps, FP32 form vs FP64 plain form" % (
             name, "" if steps else " full length", g["nx"], g["ny"], g["nz"], T),
+        "monitor": "x=%d y=%d z=%d atom %d dir %d (fields at z+1)" % (mon["x"], mon["y"], mon["z"], mon["atom"],
+                                                                     mon["dir"]),
         "monitor_rel_l2": rl2(a["rows"][:, 1:10], b["rows"][:, 1:10]),
         "monitor_rv_rel_l2": rl2(a["rows"][:, 10:12], b["rows"][:, 10:12]),
-        "rv_fp64_storage": {"monitor_rel_l2": rl2(c["rows"][:, 1:10], b["rows"][:, 1:10]),
-                            "monitor_rv_rel_l2": rl2(c["rows"][:, 10:12], b["rows"][:, 10:12]),
-                            "vs_compensated_monitor_rel_l2": rl2(a["rows"][:, 1:12], c["rows"][:, 1:12])},
+        # the first steps only: the source has injected a negligible fraction
+        # of its pulse (a Ricker wavelet peaks at t0), so the monitor holds the
+        # far tail of the numerical field, where FP32 rounding dominates
+        "monitor_rel_l2_first_70_steps": rl2(a["rows"][:70, 1:10], b["rows"][:70, 1:10]),
+        "source_peak_step": round(src["t0"] / g["dt"], 1) if src["type"] in (0, 2, 3) else None,
         "nearfield": {"component": "Ez", "plane": "z=%d" % zplane, "samples": N, "interval": interval,
                       "bin": b["k"], "bin_hz": b["k"] * fs / N, "source_hz": f,
                       "amp_rel_l2": rl2(amp32, amp64),
                       "phase_rms_rad": float(np.sqrt(np.sum(amp64 ** 2 * ph ** 2) / np.sum(amp64 ** 2))),
                       "phase_max_rad_strong": float(np.abs(ph[strong]).max())},
-        "solve_s": {"fp32": round(a["wall"], 3), "fp32_rv64": round(c["wall"], 3), "fp64": round(b["wall"], 3)},
+        "solve_s": {"fp32": round(a["wall"], 3), "fp64": round(b["wall"], 3)},
     }
+    if c:
+        out["rv_fp64_storage"] = {"monitor_rel_l2": rl2(c["rows"][:, 1:10], b["rows"][:, 1:10]),
+                                  "monitor_rv_rel_l2": rl2(c["rows"][:, 10:12], b["rows"][:, 10:12]),
+                                  "vs_compensated_monitor_rel_l2": rl2(a["rows"][:, 1:12], c["rows"][:, 1:12])}
+        out["solve_s"]["fp32_rv64"] = round(c["wall"], 3)
+    return out
 
 
 def serving(n, step_scale, seed=4, load=0.8):
@@ -261,46 +280,140 @@ def serving(n, step_scale, seed=4, load=0.8):
     }
 
 
-def cpu_baseline(budget_s=25.0):
-    """Reference fdtd.c (oracle/_ref/fdtd.x, -Ofast -flto -funroll-loops) under
-    mpirun on the host cores, on the same workload scaled down 4x per axis
-    (256x256x160 cells, same slab/cone/CPML layout), for a step count sized to
-    ~budget_s of CPU time.  Falls back to the single-thread C oracle
-    (oracle/liborc.so) when the reference binary is absent."""
+def host_info():
+    """CPU model, physical cores, the CPU share of this process, host RAM and
+    the MPI implementation, for the CPU baseline record."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        info["cpu_share"] = len(os.sched_getaffinity(0))
+    except Exception:
+        info["cpu_share"] = os.cpu_count()
+    try:
+        cores, model = set(), None
+        phys = core = None
+        for ln in open("/proc/cpuinfo"):
+            k, _, v = ln.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+        info["cpu_model"] = model
+        info["physical_cores"] = len(cores) or None
+    except Exception:
+        pass
+    try:
+        mem = {ln.split(":")[0]: int(ln.split()[1]) for ln in open("/proc/meminfo")}
+        info["mem_total_gb"] = round(mem["MemTotal"] / 1e6, 1)
+        info["mem_available_gb"] = round(mem["MemAvailable"] / 1e6, 1)
+    except Exception:
+        pass
+    try:
+        v = subprocess.run(["/opt/conda/bin/mpirun", "--version"], capture_output=True, text=True, timeout=30)
+        info["mpi"] = " ".join((v.stdout + v.stderr).split()[:6])
+    except Exception:
+        pass
+    return info
+
+
+def reference_host_bytes(spec, med):
+    """Host bytes the reference solver holds for a workload, from its data
+    layout: nine FP64 field arrays plus |E|, int32 medium ids and per-cell r/v
+    pointers (fdtd.c:261-284, 2959-3049), one heap array of 3N doubles for each
+    of r and v per lattice cell, and the CPML slabs (z slabs full-volume on the
+    ranks holding them), scaled by 1.3: the reference's own [MEM] totals were
+    1.26x (256x256x160) and 1.42x (128x128x80) of this layout count for the
+    1024x1024x640 workload scaled down, the difference being allocator
+    overhead and buffers."""
+    from php_fdtd_amd import workloads
+
+    counts, pml = workloads.cell_counts(spec, med)
+    b = med.size * (10 * 8 + 4 + 16) + 8 * 8 * pml
+    for m, mat in enumerate(spec["materials"]):
+        if mat.get("natom", 0) and counts[m]:
+            b += counts[m] * 2 * (24 * mat["natom"] + 16)
+    return 1.3 * float(b)
+
+
+def run_reference(spec, med, nsteps, ranks, timeout, log_path=None):
+    """Reference fdtd.c (oracle/_ref/fdtd.x, -Ofast -flto -funroll-loops,
+    README.md:122) under mpirun on `ranks` host cores; output_interval past the
+    run, so only the t = 0 output falls inside the loop timer.  Returns the
+    loop time of "Simulation completed in X seconds" (fdtd.c:3917), the wall
+    time with set-up, and the reference's own [MEM] total.  With log_path the
+    reference's output also streams to that file as it runs."""
     from php_fdtd_amd import workloads
 
     ref = os.path.join(ROOT, "oracle", "_ref", "fdtd.x")
-    mpirun = "/opt/conda/bin/mpirun"
+    tmp = tempfile.mkdtemp(prefix="phpfdtd_cpu_")
     try:
-        ncpu = len(os.sched_getaffinity(0))
-    except Exception:
-        ncpu = os.cpu_count() or 1
-    cores = max(1, min(16, ncpu))
-    spec, med = workloads.tip_slab(scale=4)
-    cells = med.size
-    if os.path.exists(ref) and os.path.exists(mpirun):
-        # probe 2 steps to size the sample
-        tmp = tempfile.mkdtemp(prefix="phpfdtd_cpu_")
-        try:
-            def run(nsteps):
-                sp = workloads.with_steps(spec, nsteps)
-                cfg = workloads.write_reference_case(tmp, sp, med)
-                env = dict(os.environ, OMP_NUM_THREADS="1")
-                p = subprocess.run([mpirun, "-np", str(cores), ref, cfg], cwd=tmp, capture_output=True, text=True,
-                                   timeout=600, env=env)
-                out = p.stdout
-                for ln in out.splitlines():
-                    if "Simulation completed in" in ln:
-                        return float(ln.split("in")[1].split()[0])
-                raise RuntimeError("reference run failed rc=%d: %s" % (p.returncode, (out + p.stderr)[-800:]))
-            t2 = run(2)
-            n = int(max(3, min(400, budget_s / max(t2 / 2, 1e-3))))
-            t = run(n)
-            return dict(value=cells * n / t / 1e9, unit="Gcell-updates/s", cores=cores, kind="reference",
-                        sample="reference fdtd.c (-Ofast -flto -funroll-loops) under mpirun -np %d, "
-                               "256x256x160 (the 1024x1024x640 workload scaled 1/4 per axis), %d steps, loop time %.2f s" % (cores, n, t))
-        finally:
-            shutil.rmtree(tmp, ignore_errors=True)
+        sp = workloads.with_steps(spec, nsteps)
+        cfg = workloads.write_reference_case(tmp, sp, med)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        out_path = log_path or os.path.join(tmp, "reference.out")
+        t0 = time.time()
+        with open(out_path, "w") as f:
+            p = subprocess.run(["/opt/conda/bin/mpirun", "-np", str(ranks), ref, cfg], cwd=tmp, stdout=f,
+                               stderr=subprocess.STDOUT, text=True, timeout=timeout, env=env)
+        wall = time.time() - t0
+        text = open(out_path).read()
+        loop = None
+        for ln in text.splitlines():
+            if "Simulation completed in" in ln:
+                loop = float(ln.split("in")[1].split()[0])
+        if loop is None:
+            raise RuntimeError("reference run failed rc=%d: %s" % (p.returncode, text[-800:]))
+        mem = re.findall(r"\[MEM\] Before time loop: per-rank = \S+ GB, total \(\d+ ranks\) = (\S+) GB", text)
+        return loop, wall, float(mem[-1]) if mem else None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_baseline(workload="tip_slab", full_steps=1, prefix_steps=10, timeout=1200, log_path=None):
+    """The reference solver on the host cores of this box (BASELINE.md section
+    3): the workload's own grid for `full_steps` steps when host RAM holds the
+    reference's layout of it; otherwise the workload is recorded as not
+    runnable and a labelled prefix of the 384x384x256 16-atom configuration
+    is timed instead.  Ranks: the CPU share of this process (the box allots
+    a fixed share of host cores to each GPU), at most nz.  Without the
+    reference binary the single-process C oracle is timed on a 1/8-scale
+    sample instead (kind "port")."""
+    from php_fdtd_amd import workloads
+
+    info = host_info()
+    ranks = info["cpu_share"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():  # the box's per-GPU CPU share
+        ranks = min(ranks, int(os.environ["OMP_NUM_THREADS"]))
+    ref = os.path.join(ROOT, "oracle", "_ref", "fdtd.x")
+    if os.path.exists(ref) and os.path.exists("/opt/conda/bin/mpirun"):
+        spec, med = workloads.WORKLOADS[workload]()
+        need = reference_host_bytes(spec, med)
+        avail = info.get("mem_available_gb", 0) * 1e9
+        note = None
+        if need * 1.2 > avail:
+            note = "%s not runnable: the reference's layout needs ~%.0f GB, host RAM available %.0f GB" % (
+                spec["name"], need / 1e9, avail / 1e9)
+            spec, med = workloads.lattice16_slab()
+            nsteps = prefix_steps
+        else:
+            nsteps = full_steps
+        ranks = max(1, min(ranks, spec["grid"]["nz"]))
+        loop, wall, mem = run_reference(spec, med, nsteps, ranks, timeout, log_path)
+        out = dict(value=med.size * nsteps / loop / 1e9, unit="Gcell-updates/s", cores=ranks, kind="reference",
+                   sample="reference fdtd.c (-Ofast -flto -funroll-loops) under mpirun -np %d on %s, %d step(s) "
+                          "(output_interval past the run), loop time %.2f s (fdtd.c:3917), %.1f s with set-up"
+                          % (ranks, spec["name"], nsteps, loop, wall),
+                   host=info, reference_mem_gb=mem, reference_mem_estimate_gb=round(need / 1e9, 1))
+        if note:
+            out["note"] = note
+        return out
     from oracle import orc
 
     spec, med = workloads.tip_slab(scale=8)
@@ -308,12 +421,23 @@ def cpu_baseline(budget_s=25.0):
     o = orc.Oracle(spec, med)
     t0 = time.time()
     n = 0
-    while time.time() - t0 < budget_s / 2 and n < 1000:
+    while time.time() - t0 < 12.0 and n < 1000:
         o.step(1)
         n += 1
     t = time.time() - t0
-    return dict(value=med.size * n / t / 1e9, unit="Gcell-updates/s", cores=1, kind="port",
-                sample="C oracle (FP64 restatement), 1 thread, 128x128x80 (the 1024x1024x640 workload scaled 1/8), %d steps" % n)
+    return dict(value=med.size * n / t / 1e9, unit="Gcell-updates/s", cores=ranks, kind="port", host=info,
+                sample="C oracle (FP64 restatement, OpenMP over %d threads), 128x128x80 (the 1024x1024x640 "
+                       "workload scaled 1/8), %d steps; reference binary absent" % (ranks, n))
+
+
+def _cpu_baseline_child(path, workload, log_path=None):
+    """Entry of the CPU-baseline subprocess: writes its result as JSON."""
+    try:
+        res = cpu_baseline(workload, log_path=log_path)
+    except Exception as e:  # reported, not fatal
+        res = {"value": None, "error": str(e)[:400]}
+    with open(path, "w") as f:
+        json.dump(res, f)
 
 
 def main():
@@ -325,9 +449,15 @@ def main():
                     help="tip_slab, lattice16_slab, lattice4_slab or lorentz_slab")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64", help="comma list of fp32, fp32rv64, fp64")
-    ap.add_argument("--accuracy", default="lattice4_slab",
-                    help="workload of the FP32-vs-FP64 accuracy run ('' = skip)")
-    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy run (0 = the workload's own)")
+    ap.add_argument("--accuracy", default=None,
+                    help="workload of the FP32-vs-FP64 accuracy run (default: the benchmarked workload; '' = skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=5000,
+                    help="steps of the accuracy run (0 = the workload's own; 5000 carries the 1024x1024x640 "
+                         "workload's Ricker pulse, peak at step 2624, past the monitor)")
+    ap.add_argument("--accuracy-rv64", action="store_true", help="also run the FP32 form with FP64 r, v")
+    ap.add_argument("--cpu-only", action="store_true", help="run only the CPU baseline and print it as the JSON line")
+    ap.add_argument("--cpu-log", default=None, help="file the reference's own output streams to as it runs")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving mix (0 = skip)")
     ap.add_argument("--serving-step-scale", type=float, default=0.05)
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false",
@@ -339,6 +469,13 @@ def main():
         print("bench.py: runs on one GPU only; the domain is resident on a single MI355X and no "
               "multi-GPU path is built (--gpus %d not supported)" % args.gpus, file=sys.stderr)
         return 2
+
+    if args.cpu_child:  # the CPU-baseline subprocess (no GPU use)
+        _cpu_baseline_child(args.cpu_child, args.workload, args.cpu_log)
+        return 0
+    if args.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline(args.workload, log_path=args.cpu_log)}))
+        return 0
 
     from php_fdtd_amd import _abi, workloads
 
@@ -361,8 +498,9 @@ def main():
         prec = precs[form]
         log("form %s" % form)
         res[form] = run_form(spec, med, prec, args.warmup, args.steps, prof_steps)
-        log("  %.3f ms/step (events), %.3f ms/step (wall), %.3f ms/step with a slice + monitor read every 10 steps" % (
-            res[form]["ms"], res[form]["wall_ms"], res[form]["sampled_ms"]))
+        log("  %.3f ms/step (events), %.3f ms/step (wall), %.3f ms/step with a slice + monitor read every 10 steps, "
+            "%.3f ms per sync with diagnostics" % (res[form]["ms"], res[form]["wall_ms"], res[form]["sampled_ms"],
+                                                  res[form]["sync_ms"]))
         for k in res[form]["prof"]:
             log("    %-24s %9.3f ms  %8.1f GB/s" % (k["name"], k["ms"], k["bytes"] / k["ms"] / 1e6 if k["ms"] else 0))
 
@@ -434,23 +572,30 @@ def main():
             rr = res["fp32rv64"]
             out["fp32_rv64"] = {"value": round(cells / (rr["ms"] * 1e-3) / 1e9, 4), "ms_per_step": round(rr["ms"], 4),
                                 "kernels_ms": {k["name"]: round(k["ms"], 4) for k in rr["prof"]}}
-        a, b = r["rows"], r64["rows"]
-        n = min(len(a), len(b))
-        if n:
-            num = np.linalg.norm(a[:n, 1:10] - b[:n, 1:10])
-            den = np.linalg.norm(b[:n, 1:10])
-            out["fp32_monitor_rel_l2_vs_fp64"] = float(num / den) if den else None
-    if args.accuracy:
-        out["accuracy"] = accuracy(args.accuracy, steps=args.accuracy_steps or None)
+        out["fp64"]["sync_ms"] = round(r64["sync_ms"], 3)
+    out["sync_ms"] = round(r["sync_ms"], 3)
+    # the reference on the host cores runs in a child process beside the GPU
+    # accuracy leg (it uses the host cores only); the serving leg, whose
+    # context builds use the host cores too, waits for it
+    cpu_proc, cpu_path, cpu_t0 = None, None, time.time()
+    if not args.no_cpu:
+        fd, cpu_path = tempfile.mkstemp(prefix="phpfdtd_cpu_", suffix=".json")
+        os.close(fd)
+        cpu_proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", cpu_path,
+                                     "--workload", args.workload] + (["--cpu-log", args.cpu_log] if args.cpu_log else []))
+    acc = args.workload if args.accuracy is None else args.accuracy
+    if acc:
+        out["accuracy"] = accuracy(acc, steps=args.accuracy_steps or None, rv64=args.accuracy_rv64)
+    if cpu_proc is not None:
+        cpu_proc.wait()
+        try:
+            out["cpu_baseline"] = json.load(open(cpu_path))
+        except Exception as e:  # reported, not fatal
+            out["cpu_baseline"] = {"value": None, "error": "no result from the CPU baseline process: %s" % e}
+        os.remove(cpu_path)
+        log("cpu baseline %.1f s: %s" % (time.time() - cpu_t0, out["cpu_baseline"]))
     if args.serving:
         out["serving"] = serving(args.serving, args.serving_step_scale)
-    if not args.no_cpu:
-        try:
-            t0 = time.time()
-            out["cpu_baseline"] = cpu_baseline()
-            log("cpu baseline %.1f s: %s" % (time.time() - t0, out["cpu_baseline"]))
-        except Exception as e:  # reported, not fatal
-            out["cpu_baseline"] = {"value": None, "error": str(e)[:300]}
     print(json.dumps(out))
     return 0
 

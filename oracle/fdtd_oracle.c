@@ -11,7 +11,6 @@
  *   init_pml                      fdtd.c:962-989
  *   inverse / determinant         fdtd.c:205-258
  *   inverse_dielectric_tensor     fdtd.c:555-625   (pseudo-complex)
- *   built-in materials            fdtd.c:628-715
  *   ADE coefficients              fdtd.c:2478-2528
  *   calculate_source_value        fdtd.c:1923-1947
  *   H update                      fdtd.c:3123-3231 (lagged psi)
@@ -26,11 +25,21 @@
  * and partially updated P_ADEo; it is defined, and equal to this one bit for
  * bit, only when gamma and wp2 are diagonal.
  *
- * Parity pins (tests/test_oracle.py): the reference binary built from
- * fdtd.c by oracle/Makefile (golden monitor traces, slices and full-monitor
- * rows under tests/golden/), closed forms for the CPML profile and sources,
- * the 3x3 inverse identity, and brute-force single-cell recurrences for the
- * ADE and the ion equation of motion.
+ * How each function is pinned (each check named is a test that exists):
+ *   tests/test_oracle.py, against the reference binary built from fdtd.c by
+ *   oracle/Makefile (golden runs under tests/golden/): every monitor row,
+ *   full-monitor row (all r, v) and float32 slice bit for bit, over CPML
+ *   with unequal cells, anisotropic lossy rotated media, diagonal ADE from
+ *   keys and from a switchxz table, the hBN and MoO3 lattices and the
+ *   built-in materials without CPML.  This pins init_pml, the pseudo-complex
+ *   eps^-1, the ADE coefficients and recurrence for diagonal tensors, the
+ *   sources, the H/D/E updates, the ion EOM and the sampling planes.
+ *   tests/test_oracle_pins.py, for what those runs leave open: orc_diag
+ *   against numpy and against the values the reference printed (including
+ *   its divergence report), orc_inverse3 by A A^-1 = I, the ADE
+ *   coefficients for full tensors by the Drude identity, the full-tensor
+ *   ADE recurrence against numpy's linear solve on one cell, and the ion
+ *   EOM against the eigenmode recurrences of a non-symmetric DMM.
  *
  * Layout: the reference's own, z fastest with a one-plane zero halo on each z
  * side: (i,j,k) -> (i*ny + j)*(nz+2) + k + 1  (fdtd.c:261-284).
@@ -149,44 +158,6 @@ double orc_source_value(const phpfdtd_source *s, double time) {
     default:
         return 0.0;
     }
-}
-
-/* Built-in materials 0-3 at source frequency f (fdtd.c:628-715).  Fills
- * eps_re/eps_im/sigma/use_ade of *m; other fields zero. */
-int orc_builtin_material(int id, double f, phpfdtd_material *m) {
-    memset(m, 0, sizeof(*m));
-    double d = 0.0;
-    switch (id) {
-    case 0:
-        d = 1.0;
-        break;
-    case 1: {
-        double w = f / 0.03e12;
-        double e0 = 1.5, s1 = 231, g1 = 69, w1 = 806, s2 = 866, g2 = 75, w2 = 1063;
-        double t1 = w1 * w1 - w * w, t2 = w2 * w2 - w * w;
-        double er = e0 + s1 * s1 * t1 / (t1 * t1 + w * w * g1 * g1) +
-                    s2 * s2 * t2 / (t2 * t2 + w * w * g2 * g2);
-        double ei = s1 * s1 * w * g1 / (t1 * t1 + w * w * g1 * g1) +
-                    s2 * s2 * w * g2 / (t2 * t2 + w * w * g2 * g2);
-        m->eps_re[0] = m->eps_re[4] = m->eps_re[8] = er;
-        m->eps_im[0] = m->eps_im[4] = m->eps_im[8] = ei;
-        if (er <= 0) m->use_ade = 1;
-        return 0;
-    }
-    case 2:
-        m->eps_re[0] = 8.0;
-        m->eps_re[4] = 7.0;
-        m->eps_re[8] = 5.0;
-        return 0;
-    case 3:
-        d = 9.0;
-        m->sigma = 0.1;
-        break;
-    default:
-        return -1;
-    }
-    m->eps_re[0] = m->eps_re[4] = m->eps_re[8] = d;
-    return 0;
 }
 
 /* ADE coefficients (fdtd.c:2478-2528), w = 2 pi f:
@@ -384,7 +355,10 @@ static void orc_one_step(orc_sim *s) {
         cc = (aw > 1e-30) ? sg / (kap * kap * eps0) * (bb - 1.0) / aw : 0.0;       \
     } while (0)
 
-    /* 1. H update, fdtd.c:3123-3231 */
+    /* 1. H update, fdtd.c:3123-3231.  Each (i, j, k) writes only its own H and
+     * psi and reads E, which this loop does not write, so the i iterations
+     * are independent and every value is computed exactly as in a serial run. */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx - 1; i++)
         for (int j = 0; j < ny - 1; j++)
             for (int k = 0; k < nz; k++) {
@@ -444,6 +418,8 @@ static void orc_one_step(orc_sim *s) {
 
     /* 2. D update, fdtd.c:3238-3356 */
     const double wloss = dt * g->f_source * 2.0 * 3.14159265; /* fdtd.c:3350 */
+    /* own D and psi written, H and E only read: independent i iterations */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i < nx - 1; i++)
         for (int j = 1; j < ny - 1; j++)
             for (int k = 0; k < nz; k++) {
@@ -520,7 +496,9 @@ static void orc_one_step(orc_sim *s) {
         else if (sr->component == 'z') Dz[c] += value * dt;
     }
 
-    /* 4. D -> E, ADE, ion EOM, fdtd.c:3404-3531 */
+    /* 4. D -> E, ADE, ion EOM, fdtd.c:3404-3531: every cell reads and writes
+     * only its own E, D, P and r, v: independent i iterations */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++)
             for (int k = 0; k < nz; k++) {

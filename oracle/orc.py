@@ -45,7 +45,6 @@ def lib():
         L.orc_pml_profile.argtypes = [ctypes.c_int, d, d, d, d, d, d, d, P(d), P(d), P(d)]
         L.orc_eps_inverse.argtypes = [P(d), P(d), P(d), P(d)]
         L.orc_inverse3.argtypes = [P(d), P(d)]
-        L.orc_builtin_material.argtypes = [ctypes.c_int, d, P(_abi.Material)]
         L.orc_ade_coefs.argtypes = [P(d), P(d), d, d] + [P(d)] * 6
         _LIB = L
     return _LIB

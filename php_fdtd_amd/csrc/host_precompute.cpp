@@ -242,8 +242,10 @@ int validate_grid(const phpfdtd_grid &g, std::string *err) {
         *err = "grid must be at least 3 x 3 x 1 cells";
         return PHPFDTD_EINVAL;
     }
-    if ((int64_t)g.nx * g.ny * g.nz > (int64_t)UINT32_MAX) {
-        *err = "grid exceeds 2^32 cells";
+    // slot cell indices are uint32 offsets into the padded field arrays (rows
+    // of nx rounded up to 32 cells), so the padded count must fit
+    if ((int64_t)((g.nx + 31) / 32 * 32) * g.ny * g.nz > (int64_t)UINT32_MAX) {
+        *err = "grid exceeds 2^32 cells (rows padded to a multiple of 32 cells)";
         return PHPFDTD_EINVAL;
     }
     if (!(g.dx > 0 && g.dy > 0 && g.dz > 0 && g.dt > 0 && g.eps0 > 0 && g.mu0 > 0)) {

@@ -1,4 +1,6 @@
 // kernels_aux.hip - per-step epilogue, diagnostics and sampling kernels.
+#include <algorithm>
+
 #include "phpfdtd_internal.h"
 
 namespace phpfdtd {
@@ -69,20 +71,23 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 // max |E| and max |Ex| over the physical cells (fdtd.c:3537-3552).  Values are
-// non-negative, so their IEEE bit patterns order like the values.
+// non-negative, so their IEEE bit patterns order like the values.  A
+// work-group takes whole rows of the padded arrays (the row index is
+// wavefront-uniform), so no per-element division is needed.
 template <typename real>
 __global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, const real *Ez, int nx, int ny,
                                                 int nz, int nxp, unsigned long long *out) {
-    const int64_t total = (int64_t)nxp * ny * nz;
+    const int64_t rows = (int64_t)ny * nz;
     unsigned long long me = 0, mx = 0;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        if ((int)(c % nxp) >= nx) continue;
-        const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
-        const double e = sqrt(ex * ex + ey * ey + ez * ez);
-        const unsigned long long be = __double_as_longlong(e), bx = __double_as_longlong(fabs(ex));
-        me = me > be ? me : be;
-        mx = mx > bx ? mx : bx;
+    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const int64_t base = row * nxp;
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const double ex = Ex[base + i], ey = Ey[base + i], ez = Ez[base + i];
+            const double e = sqrt(ex * ex + ey * ey + ez * ez);
+            const unsigned long long be = __double_as_longlong(e), bx = __double_as_longlong(fabs(ex));
+            me = me > be ? me : be;
+            mx = mx > bx ? mx : bx;
+        }
     }
     me = wave_max_u64(me);
     mx = wave_max_u64(mx);
@@ -92,47 +97,108 @@ __global__ __launch_bounds__(256) void reduce_e(const real *Ex, const real *Ey, 
     }
 }
 
-// max |r| over one lattice material's state (FP64 values, or the FP32 words
-// of compensated state: max |r| rounded to FP32; the compensation rows, the
-// elements of a tile from `words` on, are skipped).
-template <typename T>
-__global__ __launch_bounds__(256) void reduce_r(const T *r, int64_t n, int64_t tile, int64_t words,
-                                                unsigned long long *out) {
-    unsigned long long m = 0;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        if (q % tile >= words) continue;
-        const unsigned long long b = __double_as_longlong(fabs((double)r[q]));
-        m = m > b ? m : b;
+// (max |r| bits, smallest key among the dofs holding it): the larger value
+// wins, equal values keep the smaller key.
+__device__ __forceinline__ void pair_merge(unsigned long long &b, unsigned long long &k, unsigned long long ob,
+                                           unsigned long long ok) {
+    if (ob > b || (ob == b && ok < k)) {
+        b = ob;
+        k = ok;
     }
-    m = wave_max_u64(m);
-    if ((threadIdx.x & 63) == 0) atomicMax(out + 2, m);
 }
 
-// First argmax in the reference's loop order (i, j, k, atom, dir): among the
-// dofs equal to the maximum, the smallest key ((i*ny + j)*nz + k)*96 + q.
-// r is in the material's lattice layout (IonLayout), values as reduce_r reads them.
-template <typename T>
-__global__ __launch_bounds__(256) void argmax_r(const T *r, const uint32_t *cells, int64_t nslot, IonLayout lay,
-                                                int ny, int nz, int nxp, unsigned long long *out) {
-    const unsigned long long target = out[2];
-    const int64_t total = (nslot + 63) / 64 * lay.tile_r();
-    unsigned long long best = ~0ull;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        int64_t s;
-        int dof;
-        if (!lay.r_decode(q, nslot, &s, &dof)) continue;  // padding, compensation words
-        if ((unsigned long long)__double_as_longlong(fabs((double)r[q])) != target) continue;
+// max |r| of one lattice material and its first argmax in the reference's
+// loop order (i, j, k, atom, dir) (fdtd.c:3569-3592), in one pass over r.
+// A wavefront takes a 64-slot tile as the lattice kernel does: lane = slot,
+// one 16-byte vector per dof group, so every load is one 1 KiB row.  Values
+// are the stored values themselves: FP64, or compensated FP32 unpacked
+// exactly (word + byte).  The key of dof q of cell (i, j, k) is
+// ((i ny + j) nz + k) 96 + q, which orders like the reference's loops; as
+// there, a value must exceed the running maximum strictly (r_val >
+// local_max_r, starting from 0), so zeros and NaNs never become the argmax.
+// Each wavefront leaves its pair in part[2 w], part[2 w + 1].
+template <bool PK>
+__global__ __launch_bounds__(256) void diag_r(const void *r, const uint32_t *cells, int64_t nslot, IonLayout lay,
+                                              int ny, int nz, int nxp, unsigned long long *part) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwave = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t ntile = (nslot + 63) / 64;
+    const int ng = lay.ng(), n3 = lay.n3;
+    unsigned long long best = 0, key = ~0ull;
+    // a lane visits its slots in device (x-fastest) order, which is not the
+    // reference's (i, j, k) order: values merge by key, and only positive
+    // values count (the reference's strict r_val > local_max_r from 0)
+    auto take = [&](double v, unsigned long long k) {
+        const unsigned long long bits = __double_as_longlong(fabs(v));
+        if (v == v && bits > 0) pair_merge(best, key, bits, k);
+    };
+    for (int64_t t = wave; t < ntile; t += nwave) {
+        const int64_t s = t * 64 + lane;
+        if (s >= nslot) continue;
         const int64_t c = cells[s];
-        const int64_t i = c % nxp, j = (c / nxp) % ny, k = c / ((int64_t)nxp * ny);
-        const unsigned long long key = (unsigned long long)(((i * ny + j) * nz + k) * 96 + dof);
-        best = best < key ? best : key;
+        const int64_t ci = c % nxp, cj = (c / nxp) % ny, ck = c / ((int64_t)nxp * ny);
+        const unsigned long long base = (unsigned long long)(((ci * ny + cj) * nz + ck) * 96);
+        if (PK) {
+            // group g: words w[g][lane][0..3]; its compensation bytes are word
+            // g % 4 of lane's 16 bytes in compensation row g / 4
+            const float *tile = static_cast<const float *>(r) + t * lay.tile_r();
+            const float4 *w = reinterpret_cast<const float4 *>(tile);
+            const int4 *cb = reinterpret_cast<const int4 *>(tile + (int64_t)kPkWords * ng);
+            for (int g0 = 0; g0 < ng; g0 += 4) {
+                const int4 bv = cb[(g0 >> 2) * 64 + lane];
+                const int bws[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const int g = g0 + h;
+                    if (g >= ng) break;
+                    const float4 hv = w[g * 64 + lane];
+                    const float hw[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (4 * g + e < n3) take(pk_unpack(hw[e], (int)(int8_t)(bws[h] >> (8 * e))), base + 4 * g + e);
+                }
+            }
+        } else {
+            const double2 *w = reinterpret_cast<const double2 *>(static_cast<const double *>(r) + t * lay.tile_r());
+            for (int g = 0; g < ng; ++g) {
+                const double2 hv = w[g * 64 + lane];
+                if (2 * g < n3) take(hv.x, base + 2 * g);
+                if (2 * g + 1 < n3) take(hv.y, base + 2 * g + 1);
+            }
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(best, off, 64);
-        best = best < o ? best : o;
+        const unsigned long long ob = __shfl_xor(best, off, 64), ok = __shfl_xor(key, off, 64);
+        pair_merge(best, key, ob, ok);
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(out + 3, best);
+    if (lane == 0) {
+        part[2 * wave] = best;
+        part[2 * wave + 1] = key;
+    }
+}
+
+// Merge n wave pairs into out[2] (max |r| bits) and out[3] (its key); one
+// work-group, run after each diag_r on the same stream.
+__global__ __launch_bounds__(256) void diag_r_merge(const unsigned long long *part, int64_t n, unsigned long long *out) {
+    __shared__ unsigned long long sb[4], sk[4];
+    unsigned long long best = 0, key = ~0ull;
+    for (int64_t w = threadIdx.x; w < n; w += blockDim.x) pair_merge(best, key, part[2 * w], part[2 * w + 1]);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long ob = __shfl_xor(best, off, 64), ok = __shfl_xor(key, off, 64);
+        pair_merge(best, key, ob, ok);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sb[threadIdx.x >> 6] = best;
+        sk[threadIdx.x >> 6] = key;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = out[2], k = out[3];
+        for (int q = 0; q < 4; ++q) pair_merge(b, k, sb[q], sk[q]);
+        out[2] = b;
+        out[3] = k;
+    }
 }
 
 // Field index of element q of a plane in the reference's slice layout
@@ -223,30 +289,24 @@ void launch_fill(T *p, T v, int64_t n, hipStream_t s) {
 template <typename real>
 void launch_reduce_e(const real *Ex, const real *Ey, const real *Ez, int nx, int ny, int nz, int nxp,
                      unsigned long long *out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_e<real>, dim3(2048), dim3(256), 0, s, Ex, Ey, Ez, nx, ny, nz, nxp, out);
+    const int64_t rows = (int64_t)ny * nz;
+    hipLaunchKernelGGL(reduce_e<real>, dim3((unsigned)std::min<int64_t>(rows, 4096)), dim3(256), 0, s, Ex, Ey, Ez,
+                       nx, ny, nz, nxp, out);
 }
 
-// r: ntile tiles of layout lay; double values, or float words when f32 (compensated state).
-void launch_reduce_r(const void *r, bool f32, int64_t ntile, IonLayout lay, unsigned long long *out, hipStream_t s) {
-    const int64_t tile = lay.tile_r(), n = ntile * tile;
-    if (n <= 0) return;
-    if (f32)
-        hipLaunchKernelGGL(reduce_r<float>, dim3(1024), dim3(256), 0, s, static_cast<const float *>(r), n, tile,
-                           (int64_t)kPkWords * lay.ng(), out);
-    else
-        hipLaunchKernelGGL(reduce_r<double>, dim3(1024), dim3(256), 0, s, static_cast<const double *>(r), n, tile,
-                           tile, out);
-}
-
-void launch_argmax_r(const void *r, bool f32, const uint32_t *cells, int64_t nslot, IonLayout lay, int ny, int nz,
-                     int nxp, unsigned long long *out, hipStream_t s) {
+// max |r| and its first argmax over one lattice material (r: layout lay,
+// compensated FP32 when pk), merged into out[2], out[3]; part holds
+// kDiagWaves pairs.
+void launch_diag_r(const void *r, bool pk, const uint32_t *cells, int64_t nslot, IonLayout lay, int ny, int nz,
+                   int nxp, unsigned long long *part, unsigned long long *out, hipStream_t s) {
     if (nslot <= 0) return;
-    if (f32)
-        hipLaunchKernelGGL(argmax_r<float>, dim3(1024), dim3(256), 0, s, static_cast<const float *>(r), cells, nslot,
-                           lay, ny, nz, nxp, out);
+    const int64_t ntile = (nslot + 63) / 64;
+    const unsigned blocks = (unsigned)std::min<int64_t>((ntile + 3) / 4, kDiagWaves / 4);
+    if (pk)
+        hipLaunchKernelGGL(diag_r<true>, dim3(blocks), dim3(256), 0, s, r, cells, nslot, lay, ny, nz, nxp, part);
     else
-        hipLaunchKernelGGL(argmax_r<double>, dim3(1024), dim3(256), 0, s, static_cast<const double *>(r), cells,
-                           nslot, lay, ny, nz, nxp, out);
+        hipLaunchKernelGGL(diag_r<false>, dim3(blocks), dim3(256), 0, s, r, cells, nslot, lay, ny, nz, nxp, part);
+    hipLaunchKernelGGL(diag_r_merge, dim3(1), dim3(256), 0, s, part, (int64_t)blocks * 4, out);
 }
 
 template <typename real>

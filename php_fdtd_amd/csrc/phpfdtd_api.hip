@@ -41,8 +41,8 @@ template <typename T> void launch_fill(T *, T, int64_t, hipStream_t);
 template <typename real>
 void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
                      hipStream_t);
-void launch_reduce_r(const void *, bool, int64_t, IonLayout, unsigned long long *, hipStream_t);
-void launch_argmax_r(const void *, bool, const uint32_t *, int64_t, IonLayout, int, int, int, unsigned long long *,
+void launch_diag_r(const void *, bool, const uint32_t *, int64_t, IonLayout, int, int, int, unsigned long long *,
+                   unsigned long long *,
                      hipStream_t);
 template <typename real>
 void launch_slice(const real *, const real *, const real *, const real *, int, char, int, int, int, int, int,
@@ -51,15 +51,15 @@ void launch_slice(const real *, const real *, const real *, const real *, int, c
 
 using namespace phpfdtd;
 
-static thread_local std::string g_err;
+// Message of the last failure on this thread.  A fixed buffer, so reporting
+// an error (a host allocation failure included) never allocates or throws.
+static thread_local char g_err[1024];
 
-static int fail(int code, const char *fmt, ...) {
-    char buf[1024];
+static int fail(int code, const char *fmt, ...) noexcept {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -148,6 +148,7 @@ struct phpfdtd_ctx {
     const void *mon_r = nullptr, *mon_v = nullptr;
     int64_t mon_rc = -1, mon_vc = -1;
     unsigned long long *d_diag = nullptr;
+    unsigned long long *d_diag_part = nullptr;  // [kDiagWaves][2] wavefront pairs of the lattice diagnostics
     float *d_slice = nullptr;
     // near-field DFT of one plane
     bool dft_on = false;
@@ -242,7 +243,7 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
     l.psi_z = L2 * l.nxp * g.ny * rs;
     l.medium = l.ncellp;
     l.small = kMaxMat * sizeof(MatCoef<double>) + 3 * ((size_t)g.nx + g.ny + g.nz) * (5 * rs + 4) +
-              (size_t)g.tsteps * kMaxSrc * rs + 1024 + 64 * 1024;
+              (size_t)g.tsteps * kMaxSrc * rs + 1024 + 64 * 1024 + (size_t)kDiagWaves * 16;
     for (size_t m = 0; m < nslot.size(); ++m) {
         const size_t n = nslot[m];
         if (!n) continue;
@@ -612,7 +613,7 @@ int phpfdtd_abi_version(void) { return PHPFDTD_ABI_VERSION; }
 
 double phpfdtd_source_value(const phpfdtd_source *src, double t) { return src ? source_value(*src, t) : 0.0; }
 
-const char *phpfdtd_last_error(void) { return g_err.c_str(); }
+const char *phpfdtd_last_error(void) { return g_err; }
 
 static int footprint_bytes_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int nmat, const int32_t *medium,
                             size_t *bytes) {
@@ -787,6 +788,7 @@ static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int 
     x->ring_rows = g->tsteps;
     if ((rc = x->alloc((void **)&x->d_ring, (size_t)std::max(1, g->tsteps) * 12 * 8))) return rc;
     if ((rc = x->alloc((void **)&x->d_diag, 64))) return rc;
+    if ((rc = x->alloc((void **)&x->d_diag_part, (size_t)kDiagWaves * 2 * sizeof(unsigned long long)))) return rc;
     if ((rc = x->alloc(&x->d_src_val, x->rs))) return rc;
     if ((rc = x->alloc((void **)&x->d_src, 4 * kMaxSrc * sizeof(int)))) return rc;
     {
@@ -1055,13 +1057,11 @@ static int sync_impl(phpfdtd_ctx *x, phpfdtd_diag *d) {
     else
         launch_reduce_e<float>((float *)x->F[0], (float *)x->F[1], (float *)x->F[2], x->nx, x->ny, x->nz, x->nxp,
                                x->d_diag, x->stream);
+    // one pass over each lattice material's r: max |r| and its first argmax
     for (auto &sp : x->specials)
         if (sp.pre.natom > 0)
-            launch_reduce_r(sp.d_r, x->pk, (int64_t)(sp.cells.size() + 63) / 64, sp.lay, x->d_diag, x->stream);
-    for (auto &sp : x->specials)
-        if (sp.pre.natom > 0)
-            launch_argmax_r(sp.d_r, x->pk, sp.d_cells, (int64_t)sp.cells.size(), sp.lay, x->ny, x->nz, x->nxp,
-                            x->d_diag, x->stream);
+            launch_diag_r(sp.d_r, x->pk, sp.d_cells, (int64_t)sp.cells.size(), sp.lay, x->ny, x->nz, x->nxp,
+                          x->d_diag_part, x->d_diag, x->stream);
     unsigned long long res[4];
     HIPCHK(hipMemcpyAsync(res, x->d_diag, sizeof(res), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));

@@ -690,9 +690,11 @@ int main(int argc, char **argv) {
             const int q = d.r_atom * 3 + d.r_dir;
             if (na > 0) {
                 printf("v value: %e\n", v[q]);
-                // a = QeZM^T E - DMM r with E of the cell's (halo-offset) plane k+1, as printed by the reference
+                // a = QeZM^T E - DMM r at the cell itself: the reference indexes its
+                // halo-offset field pointers with the loop's k here (fdtd.c:3641-3645),
+                // i.e. physical plane k, unlike the monitor's k + halo (fdtd.c:3814)
                 double f9[9];
-                check(phpfdtd_read_point(ctx, d.r_i, d.r_j, d.r_k + 1, f9), "phpfdtd_read_point");
+                check(phpfdtd_read_point(ctx, d.r_i, d.r_j, d.r_k, f9), "phpfdtd_read_point");
                 const Mat &m = c.mats[med[((size_t)d.r_i * c.ny + d.r_j) * c.nz + d.r_k]];
                 double a = 0;
                 for (int e = 0; e < 3; ++e) a += m.qezm[d.r_atom * 9 + e * 3 + d.r_dir] * f9[e];

@@ -299,6 +299,24 @@ def case_builtins_nopml():
     return g, ids, [], srcs, out, mon, {}
 
 
+def case_diverge():
+    """The hBN lattice driven ~1e19 times harder from an off-centre dipole, so
+    that max |r| passes the reference's 1e-10 m guard part way through the
+    run: the reference prints its divergence report and aborts
+    (fdtd.c:3626-3667).  Every equation is linear, so nothing overflows; the
+    off-centre source keeps the argmax cell free of mirror-image ties."""
+    nx, ny, nz = 20, 20, 16
+    g = dict(nx=nx, ny=ny, nz=nz, dx=20e-9, dy=20e-9, dz=20e-9, L=3, R=1e-6, tsteps=200, interval=20)
+    ids = np.zeros((nx, ny, nz), np.int32)
+    ids[:, :, 6:10] = 4
+    files = {}
+    line, mat = lattice_material("hBN", 4, "Material_data/hBN", files, sw=False, anglez=0.5235987755982988)
+    srcs = [dict(type=3, x=7, y=12, z=4, amplitude=5e18, wavenumber=1370.0, t0=2e-15, tau=1e-15, component="z")]
+    out = dict(component="Ez", axis="x", pos=8)
+    mon = dict(x=8, y=11, z=7, atom=2, dir="y")
+    return g, ids, [line], srcs, out, mon, {4: mat}, files, True
+
+
 CASES = {
     "vacuum_cpml": case_vacuum_cpml,
     "aniso_loss": case_aniso_loss,
@@ -306,6 +324,7 @@ CASES = {
     "hbn": case_hbn,
     "moo3": case_moo3,
     "builtins_nopml": case_builtins_nopml,
+    "diverge": case_diverge,
 }
 
 
@@ -313,6 +332,7 @@ def build_case(name):
     res = CASES[name]()
     g, ids, mat_lines, srcs, out, mon, mats = res[:7]
     files = res[7] if len(res) > 7 else {}
+    aborts = res[8] if len(res) > 8 else False
     d = os.path.join(HERE, name)
     if os.path.isdir(d):
         shutil.rmtree(d)
@@ -351,6 +371,8 @@ def build_case(name):
         materials=table, sources=rsrcs, output=dict(out, interval=g["interval"]),
         monitor=dict(mon, dir={"x": 0, "y": 1, "z": 2}[mon["dir"]]),
     )
+    if aborts:
+        spec["aborts"] = True
     with open(os.path.join(d, "case.json"), "w") as f:
         json.dump(spec, f, indent=1)
 
@@ -358,9 +380,17 @@ def build_case(name):
     with open(os.path.join(d, "reference.log"), "w") as f:
         f.write(log.stdout[-20000:])
         f.write(log.stderr[-5000:])
-    if log.returncode != 0:
+    if aborts:
+        # the reference stops at MPI_Abort without flushing its monitor files:
+        # only the log (stdout, flushed) and the closed slice files are kept
+        if log.returncode == 0 or "ERROR: r field too large" not in log.stdout:
+            raise SystemExit("reference did not abort on %s" % name)
+        for fn in ("monitor_point.dat", "full_monitor.dat"):
+            p = os.path.join(d, fn)
+            if os.path.exists(p):
+                os.remove(p)
+    elif log.returncode != 0:
         raise SystemExit("reference failed on %s: rc=%d\n%s" % (name, log.returncode, log.stdout[-2000:]))
-    # drop the copied inputs that are not needed by the tests? keep all: they are the cfg's inputs
     print("%-16s ok  slices=%d" % (name, len([x for x in os.listdir(d) if x.startswith("field_slice_")])))
 
 

@@ -7,7 +7,18 @@ import re
 import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = sorted(d for d in os.listdir(GOLDEN) if os.path.exists(os.path.join(GOLDEN, d, "case.json")))
+ALL_CASES = sorted(d for d in os.listdir(GOLDEN) if os.path.exists(os.path.join(GOLDEN, d, "case.json")))
+
+
+def _aborts(name):
+    return bool(json.load(open(os.path.join(GOLDEN, name, "case.json"))).get("aborts"))
+
+
+# cases the reference ran to completion (monitor, full monitor and every slice
+# present); ABORTED: runs the reference stopped at its divergence guard
+# (fdtd.c:3626-3667), which keep only the log and the slices written before it
+CASES = [c for c in ALL_CASES if not _aborts(c)]
+ABORTED = [c for c in ALL_CASES if _aborts(c)]
 
 
 def read_medium(path):
@@ -35,8 +46,9 @@ def load(name):
     d = os.path.join(GOLDEN, name)
     spec = json.load(open(os.path.join(d, "case.json")))
     medium = read_medium(os.path.join(d, "medium_map.bin"))
-    mon = read_table(os.path.join(d, "monitor_point.dat"))
-    full = read_table(os.path.join(d, "full_monitor.dat"))
+    mon_p, full_p = os.path.join(d, "monitor_point.dat"), os.path.join(d, "full_monitor.dat")
+    mon = read_table(mon_p) if os.path.exists(mon_p) else None
+    full = read_table(full_p) if os.path.exists(full_p) else None
     slices = {}
     for p in glob.glob(os.path.join(d, "field_slice_*.bin")):
         step = int(re.search(r"_(\d{8})\.bin$", p).group(1))

@@ -165,3 +165,32 @@ def test_serving_admission_by_footprint():
     for r in reqs:
         rows, _ = serve.run_alone(r, _abi.FP32)
         assert np.array_equal(r.rows, rows)
+
+
+@pytest.mark.timeout(300)
+def test_serving_every_grid_and_material():
+    """One request of every serving grid (64^3 .. 256x256x160) with every
+    material kind (Lorentz ADE, 4-atom and 16-atom lattices): 15 requests of
+    different sizes and step counts served together in the FP32 form, each
+    result bit-identical to the same request run alone.  Step counts of 150 to
+    230 let the source's field reach every monitor (up to 36 cells away; the
+    stencil carries it one cell per step)."""
+    reqs = []
+    rng = np.random.default_rng(8)
+    for gi, shape in enumerate(workloads.SERVE_GRIDS):
+        for ki, kind in enumerate(workloads.SERVE_KINDS):
+            rid = 3 * gi + ki
+            d = dict(id=rid, shape=shape, kind=kind, wn=float(rng.uniform(600.0, 1600.0)),
+                     steps=int(rng.integers(150, 230)), gap=0.0)
+            spec, med = workloads.serving_case(d)
+            reqs.append(serve.Request(rid, spec, med, arrival=0.001 * rid))
+    print("serving all grids", [(r.spec["name"], r.steps) for r in reqs], flush=True)
+    srv = serve.Server(_abi.FP32, max_members=6, max_chunk=16)
+    srv.serve(reqs)
+    assert max(m for _, m, _ in srv.trace) > 1   # requests overlapped on the device
+    for r in reqs:
+        assert r.error is None, r.error
+        rows, _ = serve.run_alone(r, _abi.FP32)
+        assert r.rows.shape == (r.steps, 12)
+        assert np.abs(r.rows[:, 1:10]).max() > 0, r.spec["name"]
+        assert np.array_equal(r.rows, rows), r.spec["name"]

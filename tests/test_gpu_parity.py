@@ -179,8 +179,11 @@ def small_lattice_case(natom):
     return spec, med
 
 
-@pytest.mark.parametrize("natom", [1, 3, 5])
+@pytest.mark.parametrize("natom", [1, 3, 5, 20, 32])
 def test_lattice_atom_counts(natom):
+    """N = 1, 3 take exact instantiations; 5, 20 and 32 (the largest count the
+    library accepts) take the generic runtime-N kernels, 20 and 32 their
+    64-lane work-groups with up to 96 dofs."""
     spec, med = small_lattice_case(natom)
     print("lattice natom", natom, flush=True)
     forms = (_abi.FP64, _abi.FP32, _abi.FP32_RV64)

@@ -1,0 +1,287 @@
+"""Pins of the CPU oracle's functions that the golden monitor, slice and
+full-monitor comparisons (tests/test_oracle.py) leave open.
+
+Every check here comes from the mathematics or from the reference's own
+printed output, never from a second copy of the oracle's formulas:
+
+  * orc_diag (max|E|, max|Ex|, max|r| and its first argmax in (i, j, k, atom,
+    dir) order, fdtd.c:3537-3592) against numpy reductions over the oracle's
+    fields and lattice state, which the golden runs pin bit for bit, at every
+    output step of every golden case; and against the |E|_max and r_max the
+    reference printed in those same runs (reference.log, "%.2e", r_max in
+    units of 1e-10 m, fdtd.c:3562-3594).
+  * The divergence block the reference printed before aborting
+    (fdtd.c:3626-3667; golden case "diverge"): step, max r, location, atom,
+    direction, v value and a value, the last recomputed with numpy from the
+    oracle's state.
+  * orc_inverse3 (fdtd.c:205-258) by A . A^-1 = I.
+  * The ADE coefficients (fdtd.c:2478-2528) for full, non-symmetric 3x3
+    tensors by the Drude identity
+        eps_inf I - (w^2 I + i w gamma)^-1 wp2 = eps_re + i eps_im,
+    w = 2 pi f_source, which a transposed product or a sign error fails.
+  * The full-tensor ADE recurrence (the oracle's deliberate departure from
+    fdtd.c:3465-3475, which is defined only for diagonal tensors) by a
+    single-cell run against numpy's linear solve of
+        (I + gamma dt/2) P' = 2 P - (I - gamma dt/2) P_old + eps0 dt^2 wp2 E,
+        E' = (D - P') / (eps0 EPS_INF).
+  * The ion equation of motion (fdtd.c:3505-3528) by a single-cell run
+    against the per-mode 2x2 recurrences of the eigen-decomposed dynamical
+    matrix, taken non-symmetric so that a transposed DMM . r fails.
+"""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import golden_cases
+from oracle import orc
+from php_fdtd_amd import _abi
+
+EPS0 = 8.854e-12
+EPS_INF = 20.0  # fdtd.c:202
+PROGRESS = re.compile(r"^it (\d+)/(\d+) \(.*?\), t = (\S+) s,\|E\|_max = (\S+) V/m,r_max = (\S+) A,")
+
+
+def numpy_diag(o, spec, medium):
+    """max|E|, max|Ex|, max|r| and the first argmax in the reference's loop
+    order (cell in file order, then atom, then direction), from numpy."""
+    ex, ey, ez = o.field(0), o.field(1), o.field(2)
+    max_e = float(np.sqrt(ex * ex + ey * ey + ez * ez).max())
+    max_ex = float(np.abs(ex).max())
+    flat_med = medium.ravel()
+    table = np.full((flat_med.size, 96), -1.0)
+    for m, mat in enumerate(spec["materials"]):
+        n = mat["natom"]
+        if n and (flat_med == m).any():
+            table[np.flatnonzero(flat_med == m), :3 * n] = np.abs(o.state(m, 0)).reshape(-1, 3 * n)
+    max_r = max(0.0, float(table.max()))
+    idx = [-1] * 5
+    if max_r > 0:
+        first = int(np.argmax(table.ravel()))  # first occurrence in C order
+        cell, dof = divmod(first, 96)
+        i, j, k = np.unravel_index(cell, medium.shape)
+        idx = [int(i), int(j), int(k), dof // 3, dof % 3]
+    return [max_e, max_ex, max_r], idx
+
+
+def progress_lines(case_dir):
+    out = []
+    for ln in open(os.path.join(case_dir, "reference.log")):
+        m = PROGRESS.match(ln)
+        if m:
+            out.append((int(m.group(1)), m.group(4), m.group(5)))
+    return out
+
+
+@pytest.mark.parametrize("name", golden_cases.CASES + golden_cases.ABORTED)
+def test_diag_vs_numpy_and_reference_log(name):
+    c = golden_cases.load(name)
+    spec, med = c["spec"], c["medium"]
+    printed = {t: (e, r) for t, e, r in progress_lines(c["dir"])}
+    interval = spec["output"]["interval"]
+    o = orc.Oracle(spec, med)
+    checked = 0
+    nonzero_r = False
+    for t in range(spec["grid"]["tsteps"]):
+        o.step(1)
+        if t % interval:
+            continue
+        v, idx = o.diag()
+        want, want_idx = numpy_diag(o, spec, med)
+        assert list(v) == want, (t, list(v), want)
+        assert idx == want_idx, (t, idx, want_idx)
+        if t not in printed:  # the diverging case stops printing at its abort
+            break
+        e_txt, r_txt = printed[t]
+        assert "%.2e" % v[0] == e_txt, (t, v[0], e_txt)
+        assert "%.2e" % (v[2] / 1e-10) == r_txt, (t, v[2], r_txt)
+        nonzero_r |= v[2] > 0
+        checked += 1
+    assert checked == len(printed) > 0
+    if any(m["natom"] for m in spec["materials"]):
+        assert nonzero_r
+
+
+def divergence_block(case_dir):
+    """The lines fdtd.c:3626-3658 print before MPI_Abort, parsed."""
+    txt = open(os.path.join(case_dir, "reference.log")).read()
+    pat = (r"ERROR: r field too large at step (\d+)\nMax r value: (\S+)\n"
+           r"Location: i=(\d+), j=(\d+), k=(\d+) \(global k=(\d+)\)\nAtom: (\d+), Direction: (\d+)\n"
+           r"v value: (\S+)\na value: (\S+)\n")
+    m = re.search(pat, txt)
+    assert m, "no divergence block in reference.log"
+    g = m.groups()
+    return dict(step=int(g[0]), max_r=g[1], loc=[int(g[2]), int(g[3]), int(g[4])], kglobal=int(g[5]),
+                atom=int(g[6]), dir=int(g[7]), v=g[8], a=g[9])
+
+
+def test_divergence_block_vs_oracle():
+    """The oracle state at the reference's abort step reproduces every value of
+    the reference's divergence report at its printed precision; r stays below
+    1e-10 m at every earlier output step."""
+    c = golden_cases.load("diverge")
+    spec, med = c["spec"], c["medium"]
+    blk = divergence_block(c["dir"])
+    interval = spec["output"]["interval"]
+    o = orc.Oracle(spec, med)
+    for t in range(blk["step"] + 1):
+        o.step(1)
+        if t % interval == 0 and t < blk["step"]:
+            assert o.diag()[0][2] <= 1e-10, t
+    assert blk["step"] % interval == 0
+    v, idx = o.diag()
+    assert v[2] > 1e-10
+    assert "%e" % v[2] == blk["max_r"]
+    i, j, k, a, d = idx
+    assert [i, j, k] == blk["loc"] and blk["kglobal"] == k
+    assert [a, d] == [blk["atom"], blk["dir"]]
+    mat = spec["materials"][int(med[i, j, k])]
+    n = mat["natom"]
+    cell = int(np.flatnonzero(np.flatnonzero(med.ravel() == med[i, j, k]) == np.ravel_multi_index((i, j, k), med.shape))[0])
+    r = o.state(int(med[i, j, k]), 0).reshape(-1, 3 * n)[cell]
+    vv = o.state(int(med[i, j, k]), 1).reshape(-1, 3 * n)[cell]
+    q = 3 * a + d
+    assert "%e" % vv[q] == blk["v"]
+    # a = sum_e QeZM[atom][e][dir] E_e - sum_j DMM[q][j] r_j with E of the cell
+    # itself (physical plane k: fdtd.c:3641-3645 index halo-offset pointers with k)
+    E = [o.field(comp)[i, j, k] for comp in range(3)]
+    qezm = np.asarray(mat["qezm"]).reshape(n, 3, 3)
+    dmm = np.asarray(mat["dmm"]).reshape(3 * n, 3 * n)
+    acc = 0.0
+    for e in range(3):
+        acc += qezm[a, e, d] * E[e]
+    for jq in range(3 * n):
+        acc -= dmm[q, jq] * r[jq]
+    assert "%e" % acc == blk["a"]
+
+
+def test_inverse3_identity():
+    rng = np.random.default_rng(3)
+    L = orc.lib()
+    for _ in range(50):
+        a = rng.standard_normal(9) + 3 * np.eye(3).ravel()
+        inv = np.empty(9)
+        assert L.orc_inverse3(_abi.f64p(a), _abi.f64p(inv)) == 0
+        np.testing.assert_allclose(a.reshape(3, 3) @ inv.reshape(3, 3), np.eye(3), atol=1e-13)
+    sing = np.array([1.0, 2, 3, 2, 4, 6, 0, 1, 1])
+    assert L.orc_inverse3(_abi.f64p(sing), _abi.f64p(np.empty(9))) == -1
+
+
+def full_tensor_drude(rng):
+    """A full, non-symmetric eps = eps_re + i eps_im with Re eps - EPS_INF
+    negative on the diagonal (the reference's ADE condition, fdtd.c:2478-2485)."""
+    er = np.diag(rng.uniform(-60.0, -25.0, 3)) + rng.uniform(-4.0, 4.0, (3, 3)) * (1 - np.eye(3))
+    ei = np.diag(rng.uniform(2.0, 9.0, 3)) + rng.uniform(-1.0, 1.0, (3, 3)) * (1 - np.eye(3))
+    return er, ei
+
+
+def ade_coefs(er, ei, f, dt):
+    out = [np.empty(9) for _ in range(6)]
+    er_c, ei_c = np.ascontiguousarray(er.ravel()), np.ascontiguousarray(ei.ravel())
+    rc = orc.lib().orc_ade_coefs(_abi.f64p(er_c), _abi.f64p(ei_c), f, dt, *[_abi.f64p(x) for x in out])
+    assert rc == 0
+    return [x.reshape(3, 3) for x in out]  # gamma, wp2, gp, gm, gp_inv, gm_inv
+
+
+def test_ade_coefficients_reproduce_eps():
+    rng = np.random.default_rng(17)
+    dt = 3.8105117766515303e-17
+    for f in (1.5e13, 3e13, 4.2e13):
+        for _ in range(10):
+            er, ei = full_tensor_drude(rng)
+            gamma, wp2, gp, gm, gp_inv, gm_inv = ade_coefs(er, ei, f, dt)
+            w = 2 * math.pi * f
+            eps = EPS_INF * np.eye(3) - np.linalg.solve(w * w * np.eye(3) + 1j * w * gamma, wp2)
+            np.testing.assert_allclose(eps.real, er, rtol=0, atol=1e-10 * np.abs(er).max())
+            np.testing.assert_allclose(eps.imag, ei, rtol=0, atol=1e-10 * np.abs(er).max())
+            np.testing.assert_allclose(gp, np.eye(3) + gamma * dt / 2, rtol=0, atol=1e-15)
+            np.testing.assert_allclose(gm, np.eye(3) - gamma * dt / 2, rtol=0, atol=1e-15)
+            np.testing.assert_allclose(gp_inv @ gp, np.eye(3), rtol=0, atol=1e-15)
+            np.testing.assert_allclose(gm_inv @ gm, np.eye(3), rtol=0, atol=1e-15)
+
+
+def one_cell_spec(mat, f, tsteps):
+    dt = 20e-9 / (3e8 * math.sqrt(3.0)) * 0.99
+    g = dict(nx=1, ny=1, nz=1, tsteps=tsteps, dx=20e-9, dy=20e-9, dz=20e-9, dt=dt, eps0=EPS0,
+             mu0=math.pi * 4e-7, pml_thickness=0, pml_reflection=1e-6, pml_order=3.0, kappa_max=7.0,
+             alpha_max=0.05, f_source=f)
+    return dict(name="one_cell", grid=g, materials=[mat], sources=[])
+
+
+def test_full_tensor_ade_single_cell():
+    """One cell, no curl (the H and D loops skip a 1x1x1 grid), D held fixed:
+    the oracle's ADE state and E after 1, 10, 100 and 3000 steps against the
+    recurrence solved by numpy."""
+    rng = np.random.default_rng(23)
+    f = 3e13
+    er, ei = full_tensor_drude(rng)
+    mat = dict(eps_re=er.ravel().tolist(), eps_im=ei.ravel().tolist(), sigma=0.0, anglez=0.0, anglex=0.0,
+               use_ade=1, natom=0, qezv=[], qezm=[], dmm=[])
+    spec = one_cell_spec(mat, f, 4000)
+    dt = spec["grid"]["dt"]
+    gamma, wp2, _, _, _, _ = ade_coefs(er, ei, f, dt)
+    med = np.zeros((1, 1, 1), np.int32)
+    o = orc.Oracle(spec, med)
+    E = rng.standard_normal(3)
+    D = rng.standard_normal(3) * EPS0
+    P = rng.standard_normal(3) * EPS0
+    Po = P - rng.standard_normal(3) * EPS0 * 1e-2
+    for comp in range(3):
+        o.set_field(comp, np.full((1, 1, 1), E[comp]))
+        o.set_field(3 + comp, np.full((1, 1, 1), D[comp]))
+    o.set_state(0, 2, P)
+    o.set_state(0, 3, Po)
+    G = gamma * dt / 2
+    I3 = np.eye(3)
+    done = 0
+    # measured: 6e-16, 3e-14, 1.4e-12, 6e-12; a transposed gamma or wp2 is off
+    # by 4e-7 (gamma) and 1.5e-4 (wp2) after one step and by 2e-3 after 100
+    for n, tol in ((1, 1e-14), (10, 1e-13), (100, 1e-11), (3000, 1e-10)):
+        for _ in range(n - done):
+            Pn = np.linalg.solve(I3 + G, 2 * P - (I3 - G) @ Po + EPS0 * dt * dt * (wp2 @ E))
+            Po, P = P, Pn
+            E = (D - P) / EPS0 / EPS_INF
+        o.step(n - done)
+        done = n
+        got_E = np.array([o.field(comp)[0, 0, 0] for comp in range(3)])
+        for got, want in ((o.state(0, 2), P), (o.state(0, 3), Po), (got_E, E)):
+            assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want), (n, got, want)
+    # the run exercised the coupling: every component of P moved
+    assert np.all(np.abs(P - o.state(0, 3)) > 0)
+
+
+def test_ion_eom_modes_single_cell():
+    """One lattice cell with zero Born charges (no field coupling): each
+    eigenmode y of the dynamical matrix K obeys y' = y + u dt - w^2 y dt^2/2,
+    u' = u - w^2 y dt.  K is V diag(w^2) V^-1 with V non-orthogonal, so K is
+    not symmetric."""
+    rng = np.random.default_rng(29)
+    natom = 3
+    n3 = 3 * natom
+    V = np.eye(n3) + 0.3 * rng.standard_normal((n3, n3))
+    w2 = (2 * math.pi * 0.03e12 * rng.uniform(300.0, 1500.0, n3)) ** 2
+    K = V @ np.diag(w2) @ np.linalg.inv(V)
+    assert np.abs(K - K.T).max() > 1e-3 * np.abs(K).max()
+    mat = dict(eps_re=[1.0, 0, 0, 0, 1.0, 0, 0, 0, 1.0], eps_im=[0.0] * 9, sigma=0.0, anglez=0.0, anglex=0.0,
+               use_ade=0, natom=natom, qezv=[0.0] * (9 * natom), qezm=[0.0] * (9 * natom), dmm=K.ravel().tolist())
+    spec = one_cell_spec(mat, 3e13, 3000)
+    dt = spec["grid"]["dt"]
+    o = orc.Oracle(spec, np.zeros((1, 1, 1), np.int32))
+    r0 = rng.standard_normal(n3) * 1e-12
+    v0 = rng.standard_normal(n3) * 1e-2
+    o.set_state(0, 0, r0)
+    o.set_state(0, 1, v0)
+    Vi = np.linalg.inv(V)
+    y, u = Vi @ r0, Vi @ v0
+    done = 0
+    for n, tol in ((1, 1e-13), (50, 1e-12), (2500, 1e-10)):
+        for _ in range(n - done):
+            y, u = y + u * dt - 0.5 * w2 * y * dt * dt, u - w2 * y * dt
+        o.step(n - done)
+        done = n
+        r, v = V @ y, V @ u
+        assert np.linalg.norm(o.state(0, 0) - r) <= tol * np.linalg.norm(r), n
+        assert np.linalg.norm(o.state(0, 1) - v) <= tol * np.linalg.norm(v), n
+    assert np.all(o.field(0) == 0)  # no coupling to E
