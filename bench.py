@@ -170,8 +170,14 @@ def accuracy(name, interval=10, steps=None, rv64=True):
         sim.set_monitor(mon["x"], mon["y"], mon["z"], mon["atom"], mon["dir"], ref_compat=True)
         k = sim.set_dft("Ez", "z", zplane, interval, N, f, ref_compat=False)
         t0 = time.perf_counter()
-        sim.run(T)
-        sim.sync(diag=False)
+        done = 0
+        while done < T:  # in chunks, so the run reports its progress
+            n = min(1000, T - done)
+            sim.run(n)
+            sim.sync(diag=False)
+            done += n
+            if done < T:
+                log("    %s: %d of %d steps, %.1f s" % (names[prec], done, T, time.perf_counter() - t0))
         wall = time.perf_counter() - t0
         rows = sim.monitor(T)
         x, ns = sim.dft()
