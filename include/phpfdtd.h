@@ -211,6 +211,21 @@ int phpfdtd_read_ions(phpfdtd_ctx *ctx, int x, int y, int z, double *r, double *
 int phpfdtd_set_dft(phpfdtd_ctx *ctx, int comp, char axis, int pos, int ref_compat, int interval,
                     int nsamples, double f_hz, int *bin);
 
+/* phpfdtd_set_dft without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, bin and results as phpfdtd_set_dft, but the
+ * weights and the accumulator live in a static per-device workspace of the
+ * library (17 MiB on each device it runs on, held until the process ends)
+ * instead of context allocations; nothing is allocated or freed.  Limits:
+ * nsamples <= 65536 and a plane of at most 1048576 cells (a 1024 x 1024 z
+ * plane); larger requests return PHPFDTD_EINVAL.  Every call on a device
+ * shares that device's workspace and clears it: a DFT installed by this call
+ * is valid until the next call of it on the same device, so uses that
+ * overlap on one device (two contexts, or two threads) race. */
+int phpfdtd_set_dft_noalloc(phpfdtd_ctx *ctx, int comp, char axis, int pos, int ref_compat, int interval,
+                            int nsamples, double f_hz, int *bin);
+
 /* Real and imaginary parts of X in the slice layout of phpfdtd_read_slice
  * (re or im may be NULL); *n_samples (may be NULL) receives the samples
  * accumulated so far. */

@@ -181,6 +181,7 @@ def load_library(path=None):
                                       P(ctypes.c_double)]
     lib.phpfdtd_set_dft.argtypes = [c_ctx, ctypes.c_int, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, P(ctypes.c_int)]
+    lib.phpfdtd_set_dft_noalloc.argtypes = lib.phpfdtd_set_dft.argtypes
     lib.phpfdtd_read_dft.argtypes = [c_ctx, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
     c_b = ctypes.c_void_p
     lib.phpfdtd_batch_create.argtypes = [ctypes.c_int, P(c_b)]
@@ -218,7 +219,7 @@ def load_library(path=None):
 EXPORTED = [
     "phpfdtd_create", "phpfdtd_set_sources", "phpfdtd_set_monitor", "phpfdtd_run", "phpfdtd_sync",
     "phpfdtd_read_slice", "phpfdtd_read_monitor", "phpfdtd_read_point", "phpfdtd_read_ions",
-    "phpfdtd_set_dft", "phpfdtd_read_dft", "phpfdtd_batch_create", "phpfdtd_batch_add", "phpfdtd_batch_remove",
+    "phpfdtd_set_dft", "phpfdtd_set_dft_noalloc", "phpfdtd_read_dft", "phpfdtd_batch_create", "phpfdtd_batch_add", "phpfdtd_batch_remove",
     "phpfdtd_batch_run", "phpfdtd_batch_sync", "phpfdtd_batch_wait", "phpfdtd_batch_size", "phpfdtd_batch_destroy",
     "phpfdtd_read_field", "phpfdtd_write_field", "phpfdtd_count_cells", "phpfdtd_read_state",
     "phpfdtd_write_state", "phpfdtd_footprint_bytes", "phpfdtd_profile", "phpfdtd_time_steps",

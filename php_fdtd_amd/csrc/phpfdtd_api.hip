@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -919,18 +920,10 @@ static int plane_of(const phpfdtd_ctx *x, char axis, int pos, int ref_compat, si
     return 0;
 }
 
-static int set_dft_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
-                    double f_hz, int *bin) {
-    if (!x || comp < 0 || comp > 9 || interval < 1 || nsamples < 2)
-        return fail(PHPFDTD_EINVAL, "bad arguments (comp %d, interval %d, nsamples %d)", comp, interval, nsamples);
-    size_t n;
-    int p;
-    int rc = plane_of(x, axis, pos, ref_compat, &n, &p);
-    if (rc) return rc;
-    HIPCHK(hipSetDevice(x->device));
-    // bin nearest f_hz with the arithmetic of frequency_analysis.c:362-390:
-    // times t_n = (t0 + n*interval)*dt, T = t_{N-1} - t_0, fs = (N-1)/T
-    const int N = nsamples;
+// Bin of the near-field DFT: the arithmetic of frequency_analysis.c:362-390,
+// with times t_n = (t0 + n*interval)*dt, T = t_{N-1} - t_0, fs = (N-1)/T; k
+// = first argmin over i < N/2 of |i fs / N - f_hz|, -1 if none.
+static int dft_bin(const phpfdtd_ctx *x, int N, int interval, double f_hz) {
     const double dt = x->g.dt;
     const double T = (double)(x->host_step + (int64_t)(N - 1) * interval) * dt - (double)x->host_step * dt;
     const double fs = 1.0 / (T / (N - 1));
@@ -943,21 +936,30 @@ static int set_dft_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_co
             k = i;
         }
     }
-    if (k < 0) return fail(PHPFDTD_EINVAL, "Error finding target frequency index");
-    // weights exp(-2 pi i k n / N) / N, angle reduced mod N exactly in integers
-    std::vector<double2> w(N);
-    for (int q = 0; q < N; ++q) {
-        const double th = 2.0 * M_PI * (double)(((int64_t)k * q) % N) / N;
-        w[q].x = cos(th) / N;
-        w[q].y = -sin(th) / N;
-    }
-    HIPCHK(hipStreamSynchronize(x->stream));
-    x->release(x->d_dft_w);
-    x->release(x->d_dft_acc);
-    x->d_dft_w = x->d_dft_acc = nullptr;
-    x->dft_on = false;
-    if ((rc = x->upload((void **)&x->d_dft_w, w))) return rc;
-    if ((rc = x->alloc((void **)&x->d_dft_acc, n * sizeof(double2)))) return rc;
+    return k;
+}
+
+// Weight n of bin k: exp(-2 pi i k n / N) / N, the angle reduced mod N
+// exactly in integers.
+static double2 dft_weight(int k, int n, int N) {
+    const double th = 2.0 * M_PI * (double)(((int64_t)k * n) % N) / N;
+    double2 w;
+    w.x = cos(th) / N;
+    w.y = -sin(th) / N;
+    return w;
+}
+
+static int dft_check(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
+                     size_t *n, int *p) {
+    if (!x || comp < 0 || comp > 9 || interval < 1 || nsamples < 2)
+        return fail(PHPFDTD_EINVAL, "bad arguments (comp %d, interval %d, nsamples %d)", comp, interval, nsamples);
+    return plane_of(x, axis, pos, ref_compat, n, p);
+}
+
+// Point the context's DFT at weights w and a zeroed accumulator acc (n
+// plane elements) and put it in the step plan.
+static void dft_install(phpfdtd_ctx *x, int comp, char axis, int p, int ref_compat, int interval, int N, size_t n,
+                        const double2 *w, double2 *acc) {
     DftArgs &a = x->dft;
     a = DftArgs{};
     const int emag = comp == PHPFDTD_EMAG;
@@ -973,12 +975,34 @@ static int set_dft_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_co
     a.t0 = x->host_step;
     a.interval = interval;
     a.nsamp = N;
-    a.w = x->d_dft_w;
-    a.acc = x->d_dft_acc;
+    a.w = w;
+    a.acc = acc;
     a.step = x->d_step;
     x->dft_comp = comp;
     x->dft_on = true;
     touch(x);
+}
+
+static int set_dft_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
+                    double f_hz, int *bin) {
+    size_t n;
+    int p;
+    int rc = dft_check(x, comp, axis, pos, ref_compat, interval, nsamples, &n, &p);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(x->device));
+    const int N = nsamples;
+    const int k = dft_bin(x, N, interval, f_hz);
+    if (k < 0) return fail(PHPFDTD_EINVAL, "Error finding target frequency index");
+    std::vector<double2> w(N);
+    for (int q = 0; q < N; ++q) w[q] = dft_weight(k, q, N);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->release(x->d_dft_w);
+    x->release(x->d_dft_acc);
+    x->d_dft_w = x->d_dft_acc = nullptr;
+    x->dft_on = false;
+    if ((rc = x->upload((void **)&x->d_dft_w, w))) return rc;
+    if ((rc = x->alloc((void **)&x->d_dft_acc, n * sizeof(double2)))) return rc;
+    dft_install(x, comp, axis, p, ref_compat, interval, N, n, x->d_dft_w, x->d_dft_acc);
     if (bin) *bin = k;
     HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
@@ -987,6 +1011,78 @@ static int set_dft_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_co
 int phpfdtd_set_dft(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval, int nsamples,
                     double f_hz, int *bin) {
     return guarded([&] { return set_dft_impl(x, comp, axis, pos, ref_compat, interval, nsamples, f_hz, bin); });
+}
+
+// Static DFT workspace of phpfdtd_set_dft_noalloc: kDftMaxSamples weights,
+// then an accumulator of up to kDftMaxPlane plane cells (17 MiB per device,
+// held until the process ends).  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address on each device comes from
+// hipGetSymbolAddress, taken once per device on the first call there.
+constexpr int kDftMaxSamples = 65536;
+constexpr int64_t kDftMaxPlane = (int64_t)1 << 20;  // a 1024 x 1024 plane
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double2 scratch[kDftMaxSamples + kDftMaxPlane];
+static std::mutex g_scratch_mu;
+static double2 *g_scratch_dev[kMaxDevices];
+
+// Device address of the workspace on device `dev` (current on the calling
+// thread).
+static int scratch_of(int dev, double2 **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return fail(PHPFDTD_EINVAL, "device %d beyond the workspace table", dev);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PHPFDTD_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double2 *>(ptr);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+static int set_dft_noalloc_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval,
+                                int nsamples, double f_hz, int *bin) {
+    size_t n;
+    int p;
+    int rc = dft_check(x, comp, axis, pos, ref_compat, interval, nsamples, &n, &p);
+    if (rc) return rc;
+    if (nsamples > kDftMaxSamples || (int64_t)n > kDftMaxPlane)
+        return fail(PHPFDTD_EINVAL,
+                    "DFT of %d samples over %zu plane cells exceeds the static workspace (at most %d samples and "
+                    "%lld cells); use phpfdtd_set_dft",
+                    nsamples, n, kDftMaxSamples, (long long)kDftMaxPlane);
+    HIPCHK(hipSetDevice(x->device));
+    double2 *ws = nullptr;
+    if ((rc = scratch_of(x->device, &ws))) return rc;
+    const int N = nsamples;
+    const int k = dft_bin(x, N, interval, f_hz);
+    if (k < 0) return fail(PHPFDTD_EINVAL, "Error finding target frequency index");
+    // earlier work of this context may still read the workspace or the
+    // previous DFT buffers (kept: this entry point frees nothing)
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->dft_on = false;
+    double2 *w = ws, *acc = ws + kDftMaxSamples;
+    double2 chunk[256];  // weights staged through the stack, 4 KiB at a time
+    for (int q0 = 0; q0 < N; q0 += 256) {
+        const int m = std::min(256, N - q0);
+        for (int q = 0; q < m; ++q) chunk[q] = dft_weight(k, q0 + q, N);
+        HIPCHK(hipMemcpyAsync(w + q0, chunk, m * sizeof(double2), hipMemcpyHostToDevice, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
+    // the accumulator starts at zero: a fill kernel, no memset node
+    launch_fill<double>(reinterpret_cast<double *>(acc), 0.0, 2 * (int64_t)n, x->stream);
+    HIPCHK(hipGetLastError());
+    dft_install(x, comp, axis, p, ref_compat, interval, N, n, w, acc);
+    if (bin) *bin = k;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int phpfdtd_set_dft_noalloc(phpfdtd_ctx *x, int comp, char axis, int pos, int ref_compat, int interval,
+                            int nsamples, double f_hz, int *bin) {
+    return guarded(
+        [&] { return set_dft_noalloc_impl(x, comp, axis, pos, ref_compat, interval, nsamples, f_hz, bin); });
 }
 
 static int read_dft_impl(phpfdtd_ctx *x, double *re, double *im, int *n_samples) {
