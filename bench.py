@@ -93,8 +93,6 @@ def kernel_symbol(name, form):
             return "phpfdtd::ion_update_f64<%d>" % n
         return "phpfdtd::ion_update_mfma<%d, %s>" % (n, "true" if form == "fp32" else "false")
     base = name.split("[")[0]
-    if base in ("h_update", "de_update"):  # default Yee tiles: 1 row per wavefront in FP32, 2 in FP64
-        return "phpfdtd::%s<%s, %d>" % (base, real, 2 if form == "fp64" else 1)
     return "phpfdtd::%s<%s>" % (base, real)
 
 

@@ -7,11 +7,12 @@ namespace phpfdtd {
 
 // Per-step epilogue (one thread): the monitor row of fdtd.c:3806-3851 for the
 // step that just finished, then the device step counter advances.  The row is
-// written at index `step`, so graph replays need no host bookkeeping.  part 0
-// writes the whole row; part 1 only D and H (taken before the next step's H
-// update starts), part 2 t, E, R, V and advances the step.
-__device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool rest, const void *const *F, bool f64) {
+// written at index `step`, so graph replays need no host bookkeeping; step
+// s wrote E/H set (s + 1) & 1, which the row samples.  part 0 writes the
+// whole row; part 1 only D and H, part 2 t, E, R, V and advances the step.
+__device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool rest, bool f64) {
     const int64_t st = *a.step;
+    const void *const *F = a.F[(st + 1) & 1];
     if (a.mon_on && st < a.ring_rows) {
         double *row = a.ring + st * 12;
         auto val = [&](int q) -> double {
@@ -45,7 +46,7 @@ __device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool res
 template <typename real>
 __global__ void step_end(StepEndArgs a) {
     if (threadIdx.x != 0) return;
-    step_row(a, a.part != 2, a.part != 1, a.F, sizeof(real) == 8);
+    step_row(a, a.part != 2, a.part != 1, sizeof(real) == 8);
 }
 
 // step_end of every domain of a batch: work-group b handles domain b.
@@ -53,7 +54,7 @@ template <typename real>
 __global__ void step_end_batch(const StepEndArgs *args, int n) {
     if (threadIdx.x != 0 || (int)blockIdx.x >= n) return;
     const StepEndArgs a = args[blockIdx.x];
-    step_row(a, true, true, a.F, sizeof(real) == 8);
+    step_row(a, true, true, sizeof(real) == 8);
 }
 
 template <typename T>
@@ -247,7 +248,8 @@ __global__ __launch_bounds__(256) void slice_k(const real *F, const real *Ex, co
 // bin): at steps t0 + n*interval, n < nsamp, acc[q] += x_n(q) * w[n] with
 // w[n] = exp(-2 pi i k n / N) / N precomputed in FP64.  ref_float rounds the
 // sample to the float32 a slice file holds.  Runs inside the step graph
-// before the step counter advances, so it sees the step just updated.
+// before the step counter advances, so it sees the step just updated (in
+// E/H set (step + 1) & 1).
 template <typename real>
 __global__ __launch_bounds__(256) void dft_accum(DftArgs a) {
     const int64_t st = *a.step - a.t0;
@@ -257,9 +259,10 @@ __global__ __launch_bounds__(256) void dft_accum(DftArgs a) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = plane_cell(q, a.axis, a.pos, a.nx, a.ny, a.nz, a.nxp);
     if (c < 0) return;
-    const real *F = static_cast<const real *>(a.F);
-    const real *Ex = static_cast<const real *>(a.Ex), *Ey = static_cast<const real *>(a.Ey),
-               *Ez = static_cast<const real *>(a.Ez);
+    const int set = (int)((st + a.t0 + 1) & 1);
+    const real *F = static_cast<const real *>(a.F[set]);
+    const real *Ex = static_cast<const real *>(a.Ex[set]), *Ey = static_cast<const real *>(a.Ey[set]),
+               *Ez = static_cast<const real *>(a.Ez[set]);
     double x = c < a.ncell ? sample_value(F, Ex, Ey, Ez, a.emag, c) : 0.0;
     if (a.ref_float) x = (double)(float)x;
     const double2 w = a.w[n];

@@ -62,6 +62,25 @@ __device__ __forceinline__ void stpair(f64x2 v, double *base, int p) {
     __builtin_nontemporal_store(v, reinterpret_cast<f64x2 *>(base + (int64_t)p * 128));
 }
 
+// E sets of this step (double-buffered, YeeArgs): E^n is read from set
+// step & 1, the new E written to the other.
+template <typename real>
+struct SlotE {
+    const real *o[3];
+    real *n[3];
+};
+template <typename real>
+__device__ __forceinline__ SlotE<real> slot_e(const SlotArgs<real> &a) {
+    const bool odd = sld(a.step, 0) & 1;
+    SlotE<real> e;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        e.o[q] = odd ? a.E[1][q] : a.E[0][q];
+        e.n[q] = odd ? a.E[0][q] : a.E[1][q];
+    }
+    return e;
+}
+
 // ADE recurrence in the crystal frame for one slot; returns P^{n+1}.
 // FP64: P^{n+1} = Gpi (2 P - Gm P^{n-1} + W E) on (P, P^{n-1}), the
 // reference's order (fdtd.c:3463-3479).  FP32: the same recurrence in
@@ -106,8 +125,9 @@ template <typename real>
 __device__ __forceinline__ void ade_slot(const SlotArgs<real> &a, const int64_t s) {
     if (s >= a.nslot) return;
     const size_t c = a.cells[s];
+    const SlotE<real> E = slot_e(a);
     const real Dl[3] = {a.Dx[c], a.Dy[c], a.Dz[c]};
-    const real El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
+    const real El[3] = {E.o[0][c], E.o[1][c], E.o[2][c]};
     real Dc[3], Ec[3], Pn[3], En[3], Eo[3];
     rot(a.R, Dl, Dc);
     rot(a.R, El, Ec);
@@ -115,9 +135,9 @@ __device__ __forceinline__ void ade_slot(const SlotArgs<real> &a, const int64_t 
     for (int e = 0; e < 3; ++e) Dc[e] -= Pn[e];
     rot(a.A, Dc, En);
     rot_t(a.R, En, Eo);
-    a.Ex[c] = Eo[0];
-    a.Ey[c] = Eo[1];
-    a.Ez[c] = Eo[2];
+    E.n[0][c] = Eo[0];
+    E.n[1][c] = Eo[1];
+    E.n[2][c] = Eo[2];
 }
 
 // Plain form: one thread per slot, everything FP64.  NA > 0 fixes natom.
@@ -153,8 +173,9 @@ __device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const do
         }
     }
     double Dp[3] = {Dc[0] - P[0], Dc[1] - P[1], Dc[2] - P[2]};
+    const SlotE<double> E = slot_e(a);
     if (a.ade) {
-        const double El[3] = {a.Ex[c], a.Ey[c], a.Ez[c]};
+        const double El[3] = {E.o[0][c], E.o[1][c], E.o[2][c]};
         double Eo[3], Pn[3];
         rot(a.R, El, Eo);
         ade_step(a, s, Eo, Pn);
@@ -163,9 +184,9 @@ __device__ __forceinline__ void ion_f64_slot(const SlotArgs<double> &a, const do
     double Ec[3], El[3];
     rot(a.A, Dp, Ec);
     rot_t(a.R, Ec, El);
-    a.Ex[c] = El[0];
-    a.Ey[c] = El[1];
-    a.Ez[c] = El[2];
+    E.n[0][c] = El[0];
+    E.n[1][c] = El[1];
+    E.n[2][c] = El[2];
     const double dt = a.dt, hdt2 = a.half_dt2;
 #pragma unroll
     for (int p = 0; p < NMAX / 2; ++p) {
@@ -321,15 +342,16 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     const bool live = slot < a.nslot;
     size_t cell = 0;
     float Dl[3] = {0.f, 0.f, 0.f}, Eold[3] = {0.f, 0.f, 0.f};
+    const SlotE<float> E = slot_e(a);
     if (live) {
         cell = a.cells[slot];
         Dl[0] = a.Dx[cell];
         Dl[1] = a.Dy[cell];
         Dl[2] = a.Dz[cell];
         if (a.ade) {
-            Eold[0] = a.Ex[cell];
-            Eold[1] = a.Ey[cell];
-            Eold[2] = a.Ez[cell];
+            Eold[0] = E.o[0][cell];
+            Eold[1] = E.o[1][cell];
+            Eold[2] = E.o[2][cell];
         }
     }
     // 1. r^n in FP32 as LDS rows (rows 3N .. 4 nk of the last k-tile zero), P_ion.
@@ -363,9 +385,9 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
         rot(a.A, Dp, Ec);
         float El[3];
         rot_t(a.R, Ec, El);
-        a.Ex[cell] = El[0];
-        a.Ey[cell] = El[1];
-        a.Ez[cell] = El[2];
+        E.n[0][cell] = El[0];
+        E.n[1][cell] = El[1];
+        E.n[2][cell] = El[2];
     }
     __builtin_amdgcn_wave_barrier();
     // 3. -DMM . r^n per 16-slot group on MFMA

@@ -1,29 +1,40 @@
-// kernels_yee.hip - H update and fused D/E update for gfx950.
+// kernels_yee.hip - the Yee update of one time step for gfx950: H, D and the
+// E of plain cells (fdtd.c:3123-3462).
 //
-// Work decomposition: every lane owns a 16-byte vector of consecutive x cells
-// (4 cells in FP32, 2 in FP64), so a wavefront covers a 1 KiB x segment of a
-// row and every field stream is read and written in 16-byte lane accesses,
-// the width at which the HBM3E streams run fastest on this part.  A
-// wavefront owns R consecutive y rows and marches up a short chunk of z
-// planes:
-//   * z neighbour: E at k+1 (H kernel) / H at k-1 (D kernel) is carried in
-//     registers from the previous plane;
-//   * y neighbour: rows r+1 (H) / r-1 (D) come from registers, one halo row
-//     per R rows is loaded from the L2 / Infinity Cache copy;
-//   * x neighbour: the adjacent cell of the lane's vector, or element 0 (H) /
-//     the last element (D) of the neighbouring lane via a cross-lane shuffle;
-//     only the wavefront's edge lane loads one extra cell.
-// j and k are wavefront-uniform.  Hardware places work-group b on XCD
-// (b mod 8); xcd_tile() renumbers the groups so that each XCD owns one
+// Every lane owns a 16-byte vector of consecutive x cells (4 in FP32, 2 in
+// FP64), so a wavefront covers a 1 KiB x segment of one y row and every field
+// stream moves in 16-byte lane accesses.  A wavefront marches a short chunk
+// of z planes, carrying the z neighbours in registers; x neighbours are the
+// adjacent element or a cross-lane shuffle.  Hardware places work-group b on
+// XCD (b mod 8); xcd_tile() renumbers the groups so each XCD owns one
 // contiguous range of tiles and neighbouring tiles share its L2.
+//
+// E and H are double-buffered (YeeArgs): step s reads set s & 1 and writes
+// set (s + 1) & 1.  D and the D psi are updated in place.
+//
+// Two plans give the same bits:
+//   * fused (yee_update, the default): one pass.  At each plane a wavefront
+//     computes the new H of its cells, then their new D and the E of plain
+//     cells.  D needs the new H of neighbours outside the wavefront - row
+//     j-1, plane k0-1 at the start of the chunk, and cell x0-1 for lane 0 -
+//     which it recomputes from the old E, H and psi with the same code
+//     (h_cells) their owner runs; the old values cannot have been replaced by
+//     then, since new values go to the other set.  E, H and D each move once
+//     per cell and step: 18 words plus the material id.
+//   * split (h_update, then de_update): the new H is written, then read back
+//     by the D/E kernel: 21 words per cell.
+// The arithmetic is written with explicit fused multiply-adds and the file is
+// compiled without FP contraction, so the owner's H and every recomputation
+// of it round identically.
 //
 // Reference semantics (fdtd.c):
 //   H: i < nx-1, j < ny-1, all k; E above the top plane is 0 (3123-3231).
 //   D: 1 <= i <= nx-2, 1 <= j <= ny-2, all k; H below k=0 is 0 (3238-3356).
-//   CPML psi is lagged: the old psi enters the field, then psi is advanced.
-//   psi is stored pre-multiplied by dt.
+//   CPML psi is lagged: the old psi enters the field, then psi is advanced;
+//   psi is stored pre-multiplied by dt.  H and psi of cells outside the loops
+//   are never changed.
 //   D loss uses E^n in the lab frame with unrotated eps_im (3349-3353).
-//   Source: D[comp] += value*dt after the curl and loss (3363-3396).
+//   Source: D[comp] += value*dt after the curl and loss (3363-3396), on any row.
 //   E: every cell (3404-3462); cells of ADE/lattice materials are left to the
 //   slot kernels (kernels_slot.hip).
 #include "phpfdtd_internal.h"
@@ -37,7 +48,6 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks, int nblocks_pad) {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
 
 template <typename real>
 struct Vec;
@@ -54,13 +64,27 @@ struct Vec<double> {
     typedef unsigned short ids;  // 2 material ids
 };
 
-template <typename V>
-__device__ __forceinline__ V vld(const V *p) {
-    return __builtin_nontemporal_load(p);
+template <typename V, typename T>
+__device__ __forceinline__ V ld(const T *p) {
+    return *reinterpret_cast<const V *>(p);
 }
+template <typename V, typename T>
+__device__ __forceinline__ void st(V v, T *p) {
+    *reinterpret_cast<V *>(p) = v;
+}
+// streams touched once per step
+template <typename V, typename T>
+__device__ __forceinline__ V ldnt(const T *p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+}
+template <typename V, typename T>
+__device__ __forceinline__ void stnt(V v, T *p) {
+    __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+}
+
 template <typename V>
-__device__ __forceinline__ void vst(V v, V *p) {
-    __builtin_nontemporal_store(v, p);
+__device__ __forceinline__ V vfma(V a, V b, V c) {
+    return __builtin_elementwise_fma(a, b, c);
 }
 
 template <typename real>
@@ -72,179 +96,674 @@ __device__ __forceinline__ real shfl_up1(real v) {
     return __shfl_up(v, 1, 64);
 }
 
-// H update of tile t of one domain.
-template <typename real, int R>
-__device__ __forceinline__ void h_tile(const YeeArgs<real> &a, const int t) {
-    constexpr int VN = Vec<real>::n;
+// The vector of the i+1 neighbours: elements 1.. of v, then `next`.
+template <typename V, typename real>
+__device__ __forceinline__ V next_x(V v, real next) {
+    constexpr int n = sizeof(V) / sizeof(real);
+    V w;
+#pragma unroll
+    for (int e = 0; e < n - 1; ++e) w[e] = v[e + 1];
+    w[n - 1] = next;
+    return w;
+}
+// The vector of the i-1 neighbours: `prev`, then elements ..n-2 of v.
+template <typename V, typename real>
+__device__ __forceinline__ V prev_x(V v, real prev) {
+    constexpr int n = sizeof(V) / sizeof(real);
+    V w;
+    w[0] = prev;
+#pragma unroll
+    for (int e = 1; e < n; ++e) w[e] = v[e - 1];
+    return w;
+}
+
+enum : int { HX = 1, HY = 2, HZ = 4 };
+
+// CPML coefficients of one y row or z plane, one scalar load.
+template <typename real>
+using AxC = AxRow<real>;
+template <typename real>
+__device__ __forceinline__ AxC<real> axc(const AxRow<real> *t, int i) {
+    static_assert(sizeof(AxRow<real>) % 4 == 0, "rows are read as dwords");
+    typedef const __attribute__((address_space(4))) uint32_t *cptr;
+    const cptr src = (cptr)(t + i);
+    AxC<real> c;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&c);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(AxRow<real>) / 4); ++w) dst[w] = src[w];
+    return c;
+}
+
+// The VN x cells of one lane vector, first cell x0.  The x coefficient
+// tables (padded to nxp) are read where they are used, so they hold no
+// registers across the z march.
+template <typename real>
+struct LaneX {
+    int sl0;                   // x psi slab index of the first cell, -1 outside the CPML
+    int xr;                    // in-row offset (lanes past the padded row read cell 0)
+    bool vx;                   // FP32: the cells are consecutive slots of one aligned slab vector
+    bool hpsi, dpsi;           // a cell with an x psi slot inside the H / D loop
+    bool hon[Vec<real>::n];    // inside the H loop: i < nx-1
+    bool don[Vec<real>::n];    // inside the D loop: 1 <= i <= nx-2
+};
+template <typename real>
+__device__ __forceinline__ LaneX<real> lane_x(const YeeArgs<real> &a, int x0) {
     typedef typename Vec<real>::type V;
-    const int bx = t % a.nbx;
-    const int rest = t / a.nbx;
-    const int by = rest % a.nby;
-    const int bz = rest / a.nby;
-    const int lane = threadIdx.x;
-    const int x0 = (bx * 64 + lane) * VN;  // first cell of this lane
-    const int j0 = uniform((by * kBlockY + threadIdx.y) * R);
-    const int nr = min(R, a.ny - 1 - j0);  // H rows j < ny-1 of this wavefront
-    if (nr <= 0 || uniform(bx * 64 * VN) >= a.nx - 1) return;
-    const bool lane_on = x0 < a.nx - 1;  // lanes past the row end still load and shuffle
-    const int xr = x0 < a.nxp ? x0 : 0;  // in-row offset (lanes past the padded row read cell 0)
-    const int k0 = uniform(a.kbase + bz * a.kchunk);
-    const int k1 = min(k0 + a.kchunk, a.kend);
+    constexpr int VN = Vec<real>::n;
+    LaneX<real> l;
+    l.xr = x0 < a.nxp ? x0 : 0;
+    const int sl0 = a.ax.slot[l.xr];
+    l.sl0 = sl0;
+    bool vx = true;
+    l.hpsi = l.dpsi = false;
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+        const int sl = a.ax.slot[l.xr + e];
+        const int i = x0 + e;
+        vx = vx && sl == sl0 + e;
+        l.hon[e] = i < a.nx - 1;
+        l.don[e] = i >= 1 && i <= a.nx - 2;
+        l.hpsi |= sl >= 0 && l.hon[e];
+        l.dpsi |= sl >= 0 && l.don[e];
+    }
+    // the vector psi path: FP32 only (2.9% off the FP32 step, neutral in FP64
+    // where it costs registers)
+    l.vx = vx && sizeof(real) == 4 && (2 * a.L) % VN == 0 && sl0 >= 0 && sl0 % VN == 0;
+    return l;
+}
+
+// E and H sets of this step: o read (set step & 1), n written.
+template <typename real>
+struct Sets {
+    const real *Eo[3], *Ho[3], *pHo[6];
+    real *En[3], *Hn[3], *pHn[6];
+};
+template <typename real>
+__device__ __forceinline__ Sets<real> sets(const YeeArgs<real> &a, int64_t step) {
+    const bool odd = step & 1;
+    Sets<real> s;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        s.Eo[q] = odd ? a.E[1][q] : a.E[0][q];
+        s.En[q] = odd ? a.E[0][q] : a.E[1][q];
+        s.Ho[q] = odd ? a.H[1][q] : a.H[0][q];
+        s.Hn[q] = odd ? a.H[0][q] : a.H[1][q];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        s.pHo[q] = odd ? a.pH[1][q] : a.pH[0][q];
+        s.pHn[q] = odd ? a.pH[0][q] : a.pH[1][q];
+    }
+    return s;
+}
+
+// New H (MASK: components) of lane vector lx in row j, plane k, from the old
+// E, H and psi (fdtd.c:3123-3231).  E inputs: e? at (j, k); exk, eyk at
+// (j, k+1); exj, ezj at (j+1, k); eyi, ezi at the i+1 neighbours.  With
+// WRITE (the cells' owner, lane holding cells: `store`) the advanced psi goes
+// to set n; a recomputation by a neighbour writes nothing.  Cells outside the
+// H loop keep H and psi.  The psi terms enter in x, y, z order.
+template <typename real, int MASK, bool WRITE>
+__device__ __forceinline__ void h_cells(const YeeArgs<real> &a, const Sets<real> &s, const LaneX<real> &lx,
+                                        const AxC<real> &yc, const AxC<real> &zc, int j, int k, bool store,
+                                        typename Vec<real>::type ex, typename Vec<real>::type ey,
+                                        typename Vec<real>::type ez, typename Vec<real>::type exk,
+                                        typename Vec<real>::type eyk, typename Vec<real>::type exj,
+                                        typename Vec<real>::type ezj, typename Vec<real>::type eyi,
+                                        typename Vec<real>::type ezi, typename Vec<real>::type &hx,
+                                        typename Vec<real>::type &hy, typename Vec<real>::type &hz) {
+    typedef typename Vec<real>::type V;
+    constexpr int VN = Vec<real>::n;
+    const size_t nxp = a.nxp;
+    const int L2 = 2 * a.L;
+    const V dyEz = ezj - ez, dzEy = eyk - ey, dxEz = ezi - ez;
+    const V dzEx = exk - ex, dxEy = eyi - ey, dyEx = exj - ex;
+    V nhx = hx, nhy = hy, nhz = hz;
+    const V khx = (MASK & (HY | HZ)) ? ld<V>(a.ax.kh + lx.xr) : V(0);
+    if (MASK & HX) nhx = vfma(V(-yc.kh), dyEz, vfma(V(zc.kh), dzEy, hx));
+    if (MASK & HY) nhy = vfma(V(-zc.kh), dzEx, vfma(khx, dxEz, hy));
+    if (MASK & HZ) nhz = vfma(-khx, dxEy, vfma(V(yc.kh), dyEx, hz));
+    if (MASK & (HY | HZ)) {  // x slabs: psi_Hxy (Hy), psi_Hxz (Hz)
+        if (lx.vx) {
+            const size_t q = ((size_t)k * a.ny + j) * L2 + lx.sl0;
+            const V b = ld<V>(a.ax.b + lx.xr), c = ld<V>(a.ax.ch + lx.xr);
+            if (MASK & HY) {
+                const V o = ld<V>(s.pHo[0] + q);
+                nhy += o;
+                V n = vfma(b, o, c * dxEz);
+#pragma unroll
+                for (int e = 0; e < VN; ++e)
+                    if (!lx.hon[e]) n[e] = o[e];
+                if (WRITE && store) st(n, s.pHn[0] + q);
+            }
+            if (MASK & HZ) {
+                const V o = ld<V>(s.pHo[1] + q);
+                nhz += o;
+                V n = vfma(b, o, -(c * dxEy));
+#pragma unroll
+                for (int e = 0; e < VN; ++e)
+                    if (!lx.hon[e]) n[e] = o[e];
+                if (WRITE && store) st(n, s.pHn[1] + q);
+            }
+        } else if (lx.hpsi) {
+#pragma unroll
+            for (int e = 0; e < VN; ++e) {
+                const int i = lx.xr + e;
+                const int px = a.ax.slot[i];
+                if (px < 0 || !lx.hon[e]) continue;
+                const size_t q = ((size_t)k * a.ny + j) * L2 + px;
+                const real b = a.ax.b[i], c = a.ax.ch[i];
+                if (MASK & HY) {
+                    const real o = s.pHo[0][q];
+                    nhy[e] += o;
+                    if (WRITE) s.pHn[0][q] = fma(b, o, c * dxEz[e]);
+                }
+                if (MASK & HZ) {
+                    const real o = s.pHo[1][q];
+                    nhz[e] += o;
+                    if (WRITE) s.pHn[1][q] = fma(b, o, -(c * dxEy[e]));
+                }
+            }
+        }
+    }
+    if ((MASK & (HX | HZ)) && yc.p >= 0) {  // y slabs: psi_Hyx (Hx), psi_Hyz (Hz)
+        const size_t q = ((size_t)k * L2 + yc.p) * nxp + lx.xr;
+        if (MASK & HX) {
+            const V o = ld<V>(s.pHo[2] + q);
+            nhx += o;
+            V n = vfma(V(yc.b), o, -(V(yc.ch) * dyEz));
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+                if (!lx.hon[e]) n[e] = o[e];
+            if (WRITE && store) st(n, s.pHn[2] + q);
+        }
+        if (MASK & HZ) {
+            const V o = ld<V>(s.pHo[3] + q);
+            nhz += o;
+            V n = vfma(V(yc.b), o, V(yc.ch) * dyEx);
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+                if (!lx.hon[e]) n[e] = o[e];
+            if (WRITE && store) st(n, s.pHn[3] + q);
+        }
+    }
+    if ((MASK & (HX | HY)) && zc.p >= 0) {  // z slabs: psi_Hzx (Hx), psi_Hzy (Hy)
+        const size_t q = ((size_t)zc.p * a.ny + j) * nxp + lx.xr;
+        if (MASK & HX) {
+            const V o = ld<V>(s.pHo[4] + q);
+            nhx += o;
+            V n = vfma(V(zc.b), o, V(zc.ch) * dzEy);
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+                if (!lx.hon[e]) n[e] = o[e];
+            if (WRITE && store) st(n, s.pHn[4] + q);
+        }
+        if (MASK & HY) {
+            const V o = ld<V>(s.pHo[5] + q);
+            nhy += o;
+            V n = vfma(V(zc.b), o, -(V(zc.ch) * dzEx));
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+                if (!lx.hon[e]) n[e] = o[e];
+            if (WRITE && store) st(n, s.pHn[5] + q);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < VN; ++e)
+        if (!lx.hon[e]) {
+            nhx[e] = hx[e];
+            nhy[e] = hy[e];
+            nhz[e] = hz[e];
+        }
+    hx = nhx;
+    hy = nhy;
+    hz = nhz;
+}
+
+// New Hy, Hz of the single cell (i, j, k), i < nx-1, from the old E, H and
+// psi: what h_cells computes for that element (each vector operation there
+// rounds per element exactly as the scalar one here), for lane 0's i-1
+// neighbour.  ezi, eyi: Ez, Ey of cell i+1.
+template <typename real>
+__device__ __forceinline__ void h_cell_yz(const YeeArgs<real> &a, const Sets<real> &s, const AxC<real> &yc,
+                                          const AxC<real> &zc, int i, int j, int k, size_t c, bool top,
+                                          real ezi, real eyi, real &hy, real &hz) {
     const size_t nxp = a.nxp;
     const size_t plane = nxp * a.ny;
     const int L2 = 2 * a.L;
-    const bool edge = lane == 63;
-    const bool edge_in = x0 + VN < a.nxp;  // the cell after this lane's vector exists
-
-    // per-cell x coefficients (tables padded to nxp)
-    const V khx = *reinterpret_cast<const V *>(a.ax.kh + xr);
-    int px[VN];
-    bool vx = true;  // x-PML lane whose VN cells are consecutive slots of one aligned slab vector
-    const int sl0 = a.ax.slot[xr];
-#pragma unroll
-    for (int e = 0; e < VN; ++e) {
-        const int sl = a.ax.slot[xr + e];
-        px[e] = x0 + e < a.nx - 1 ? sl : -1;
-        vx = vx && sl == sl0 + e;
+    const real ex = s.Eo[0][c], ey = s.Eo[1][c], ez = s.Eo[2][c];
+    const real exk = top ? real(0) : s.Eo[0][c + plane];
+    const real exj = s.Eo[0][c + nxp];
+    const real kh = a.ax.kh[i];
+    const real dxEz = ezi - ez, dzEx = exk - ex, dxEy = eyi - ey, dyEx = exj - ex;
+    real nhy = fma(-zc.kh, dzEx, fma(kh, dxEz, s.Ho[1][c]));
+    real nhz = fma(-kh, dxEy, fma(yc.kh, dyEx, s.Ho[2][c]));
+    const int px = a.ax.slot[i];
+    if (px >= 0) {
+        const size_t q = ((size_t)k * a.ny + j) * L2 + px;
+        nhy += s.pHo[0][q];
+        nhz += s.pHo[1][q];
     }
-    // such lanes take the vector psi path, the others go cell by cell; FP32
-    // only (2.9% off the FP32 step, neutral in FP64 where it costs registers)
-    vx = vx && sizeof(real) == 4 && L2 % VN == 0 && sl0 >= 0 && sl0 % VN == 0;
-    real khy[R], byr[R], chr_[R];
-    int py[R];
+    if (yc.p >= 0) nhz += s.pHo[3][((size_t)k * L2 + yc.p) * nxp + i];
+    if (zc.p >= 0) nhy += s.pHo[5][((size_t)zc.p * a.ny + j) * nxp + i];
+    hy = nhy;
+    hz = nhz;
+}
+
+// New D of lane vector lx in row j (inside the D loop), plane k
+// (fdtd.c:3238-3353): curl of the new H (h? at the cells, hxj, hzj at row
+// j-1, hxk, hyk at plane k-1, hyi, hzi at the i-1 neighbours), psi advanced
+// in place, then the eps_im loss with E^n (e?) for lossy cells.  Cells
+// outside the D loop keep D.
+template <typename real>
+__device__ __forceinline__ void d_cells(const YeeArgs<real> &a, const LaneX<real> &lx, const AxC<real> &yc,
+                                        const AxC<real> &zc, int j, int k, bool store, const MatCoef<real> *mats,
+                                        const int *mid, const int *fl, bool anyloss, typename Vec<real>::type hx,
+                                        typename Vec<real>::type hy, typename Vec<real>::type hz,
+                                        typename Vec<real>::type hxj, typename Vec<real>::type hzj,
+                                        typename Vec<real>::type hxk, typename Vec<real>::type hyk,
+                                        typename Vec<real>::type hyi, typename Vec<real>::type hzi,
+                                        typename Vec<real>::type ex, typename Vec<real>::type ey,
+                                        typename Vec<real>::type ez, typename Vec<real>::type &dx,
+                                        typename Vec<real>::type &dy, typename Vec<real>::type &dz) {
+    typedef typename Vec<real>::type V;
+    constexpr int VN = Vec<real>::n;
+    const size_t nxp = a.nxp;
+    const int L2 = 2 * a.L;
+    const V dyHz = hz - hzj, dzHy = hy - hyk, dzHx = hx - hxk;
+    const V dxHz = hz - hzi, dxHy = hy - hyi, dyHx = hx - hxj;
+    const V kdx = ld<V>(a.ax.kd + lx.xr);
+    V ndx = vfma(V(-zc.kd), dzHy, vfma(V(yc.kd), dyHz, dx));
+    V ndy = vfma(-kdx, dxHz, vfma(V(zc.kd), dzHx, dy));
+    V ndz = vfma(V(-yc.kd), dyHx, vfma(kdx, dxHy, dz));
+    if (lx.vx) {  // x slabs: psi_Dxy (Dy), psi_Dxz (Dz)
+        const size_t q = ((size_t)k * a.ny + j) * L2 + lx.sl0;
+        const V b = ld<V>(a.ax.b + lx.xr), c = ld<V>(a.ax.cd + lx.xr);
+        const V oy = ld<V>(a.pDxy + q), oz = ld<V>(a.pDxz + q);
+        ndy += oy;
+        ndz += oz;
+        V ny = vfma(b, oy, -(c * dxHz)), nz = vfma(b, oz, c * dxHy);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int j = min(j0 + r, a.ny - 1);
-        khy[r] = sld(a.ay.kh, j);
-        py[r] = sld(a.ay.slot, j);
-        byr[r] = sld(a.ay.b, j);
-        chr_[r] = sld(a.ay.ch, j);
+        for (int e = 0; e < VN; ++e)
+            if (!lx.don[e]) {
+                ny[e] = oy[e];
+                nz[e] = oz[e];
+            }
+        if (store) {
+            st(ny, a.pDxy + q);
+            st(nz, a.pDxz + q);
+        }
+    } else if (lx.dpsi) {
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            const int i = lx.xr + e;
+            const int px = a.ax.slot[i];
+            if (px < 0 || !lx.don[e]) continue;
+            const size_t q = ((size_t)k * a.ny + j) * L2 + px;
+            const real b = a.ax.b[i], c = a.ax.cd[i];
+            const real oy = a.pDxy[q], oz = a.pDxz[q];
+            ndy[e] += oy;
+            ndz[e] += oz;
+            a.pDxy[q] = fma(b, oy, -(c * dxHz[e]));
+            a.pDxz[q] = fma(b, oz, c * dxHy[e]);
+        }
     }
-    bool anypx = false;
+    if (yc.p >= 0) {  // y slabs: psi_Dyx (Dx), psi_Dyz (Dz)
+        const size_t q = ((size_t)k * L2 + yc.p) * nxp + lx.xr;
+        const V ox = ld<V>(a.pDyx + q), oz = ld<V>(a.pDyz + q);
+        ndx += ox;
+        ndz += oz;
+        V nx = vfma(V(yc.b), ox, V(yc.cd) * dyHz), nz = vfma(V(yc.b), oz, -(V(yc.cd) * dyHx));
 #pragma unroll
-    for (int e = 0; e < VN; ++e) anypx |= px[e] >= 0;
+        for (int e = 0; e < VN; ++e)
+            if (!lx.don[e]) {
+                nx[e] = ox[e];
+                nz[e] = oz[e];
+            }
+        if (store) {
+            st(nx, a.pDyx + q);
+            st(nz, a.pDyz + q);
+        }
+    }
+    if (zc.p >= 0) {  // z slabs: psi_Dzx (Dx), psi_Dzy (Dy)
+        const size_t q = ((size_t)zc.p * a.ny + j) * nxp + lx.xr;
+        const V ox = ld<V>(a.pDzx + q), oy = ld<V>(a.pDzy + q);
+        ndx += ox;
+        ndy += oy;
+        V nx = vfma(V(zc.b), ox, -(V(zc.cd) * dzHy)), ny = vfma(V(zc.b), oy, V(zc.cd) * dzHx);
+#pragma unroll
+        for (int e = 0; e < VN; ++e)
+            if (!lx.don[e]) {
+                nx[e] = ox[e];
+                ny[e] = oy[e];
+            }
+        if (store) {
+            st(nx, a.pDzx + q);
+            st(ny, a.pDzy + q);
+        }
+    }
+    if (anyloss) {
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            if (!(fl[e] & MF_LOSS)) continue;
+            const real *Lk = mats[mid[e]].Lk;
+            ndx[e] -= fma(Lk[0], ex[e], fma(Lk[1], ey[e], Lk[2] * ez[e]));
+            ndy[e] -= fma(Lk[3], ex[e], fma(Lk[4], ey[e], Lk[5] * ez[e]));
+            ndz[e] -= fma(Lk[6], ex[e], fma(Lk[7], ey[e], Lk[8] * ez[e]));
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < VN; ++e)
+        if (lx.don[e]) {
+            dx[e] = ndx[e];
+            dy[e] = ndy[e];
+            dz[e] = ndz[e];
+        }
+}
 
-    size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + xr;
-    V ex[R + 1], ey[R];
+// Soft sources of row j, plane k at step `st` (fdtd.c:3363-3396), added in
+// source order.  Each element is taken as a scalar: an element index that
+// depends on the source position would make the compiler index the vectors
+// dynamically, through private memory.
+template <typename real>
+__device__ __forceinline__ void inject(const YeeArgs<real> &a, int x0, int j, int k, int64_t st,
+                                       typename Vec<real>::type &dx, typename Vec<real>::type &dy,
+                                       typename Vec<real>::type &dz) {
+    constexpr int VN = Vec<real>::n;
+    if (a.nsrc == 0) return;
 #pragma unroll
-    for (int r = 0; r <= R; ++r)
-        if (r <= nr) ex[r] = *reinterpret_cast<const V *>(a.Ex + c0 + r * nxp);
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (r < nr) ey[r] = *reinterpret_cast<const V *>(a.Ey + c0 + r * nxp);
-
-    for (int k = k0; k < k1; ++k, c0 += plane) {
-        V exn[R + 1], eyn[R], ez[R + 1];
-        const bool top = k + 1 >= a.nz;  // E above the top plane is the zero halo
-#pragma unroll
-        for (int r = 0; r <= R; ++r)
-            if (r <= nr) {
-                exn[r] = top ? V(0) : *reinterpret_cast<const V *>(a.Ex + c0 + plane + r * nxp);
-                ez[r] = *reinterpret_cast<const V *>(a.Ez + c0 + r * nxp);
-            }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (r < nr) eyn[r] = top ? V(0) : *reinterpret_cast<const V *>(a.Ey + c0 + plane + r * nxp);
-        const real khz = sld(a.az.kh, k);
-        const int pz = sld(a.az.slot, k);
-        const real bzv = sld(a.az.b, k), chz = sld(a.az.ch, k);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r >= nr) continue;
-            const size_t c = c0 + r * nxp;
-            const int j = j0 + r;
-            // x neighbours: element e+1 of the vector, element 0 of the next lane
-            real ez_n = shfl_down1(ez[r][0]), ey_n = shfl_down1(ey[r][0]);
-            if (edge && edge_in) {
-                ez_n = a.Ez[c + VN];
-                ey_n = a.Ey[c + VN];
-            }
-            V ezip, eyip;
-#pragma unroll
-            for (int e = 0; e < VN - 1; ++e) {
-                ezip[e] = ez[r][e + 1];
-                eyip[e] = ey[r][e + 1];
-            }
-            ezip[VN - 1] = ez_n;
-            eyip[VN - 1] = ey_n;
-            const V dyEz = ez[r + 1] - ez[r], dzEy = eyn[r] - ey[r];
-            const V dxEz = ezip - ez[r], dzEx = exn[r] - ex[r];
-            const V dxEy = eyip - ey[r], dyEx = ex[r + 1] - ex[r];
-            V hx = vld(reinterpret_cast<const V *>(a.Hx + c));
-            V hy = vld(reinterpret_cast<const V *>(a.Hy + c));
-            V hz = vld(reinterpret_cast<const V *>(a.Hz + c));
-            V nhx = hx + khz * dzEy - khy[r] * dyEz;
-            V nhy = hy + khx * dxEz - khz * dzEx;
-            V nhz = hz + khy[r] * dyEx - khx * dxEy;
-            if (vx) {
-                V *sxyp = reinterpret_cast<V *>(a.pHxy + ((size_t)k * a.ny + j) * L2 + sl0);
-                V *sxzp = reinterpret_cast<V *>(a.pHxz + ((size_t)k * a.ny + j) * L2 + sl0);
-                const V sxy = *sxyp, sxz = *sxzp;
-                const V bxv = *reinterpret_cast<const V *>(a.ax.b + xr);
-                const V chxv = *reinterpret_cast<const V *>(a.ax.ch + xr);
-                nhy += sxy;
-                nhz += sxz;
-                V nsxy = bxv * sxy + chxv * dxEz, nsxz = bxv * sxz - chxv * dxEy;
-#pragma unroll
-                for (int e = 0; e < VN; ++e)
-                    if (x0 + e >= a.nx - 1) { nsxy[e] = sxy[e]; nsxz[e] = sxz[e]; }
-                if (lane_on) { *sxyp = nsxy; *sxzp = nsxz; }
-            } else if (anypx) {
-#pragma unroll
-                for (int e = 0; e < VN; ++e) {
-                    if (px[e] < 0) continue;
-                    const int i = xr + e;
-                    const size_t q = ((size_t)k * a.ny + j) * L2 + px[e];
-                    const real sxy = a.pHxy[q], sxz = a.pHxz[q];
-                    nhy[e] += sxy;
-                    nhz[e] += sxz;
-                    a.pHxy[q] = a.ax.b[i] * sxy + a.ax.ch[i] * dxEz[e];
-                    a.pHxz[q] = a.ax.b[i] * sxz - a.ax.ch[i] * dxEy[e];
-                }
-            }
-            if (py[r] >= 0) {
-                V *syxp = reinterpret_cast<V *>(a.pHyx + ((size_t)k * L2 + py[r]) * nxp + xr);
-                V *syzp = reinterpret_cast<V *>(a.pHyz + ((size_t)k * L2 + py[r]) * nxp + xr);
-                const real byv = byr[r], chy = chr_[r];
-                const V syx = *syxp, syz = *syzp;
-                nhx += syx;
-                nhz += syz;
-                V nsyx = byv * syx - chy * dyEz, nsyz = byv * syz + chy * dyEx;
-#pragma unroll
-                for (int e = 0; e < VN; ++e)
-                    if (x0 + e >= a.nx - 1) { nsyx[e] = syx[e]; nsyz[e] = syz[e]; }
-                if (lane_on) { *syxp = nsyx; *syzp = nsyz; }
-            }
-            if (pz >= 0) {
-                V *szxp = reinterpret_cast<V *>(a.pHzx + ((size_t)pz * a.ny + j) * nxp + xr);
-                V *szyp = reinterpret_cast<V *>(a.pHzy + ((size_t)pz * a.ny + j) * nxp + xr);
-                const V szx = *szxp, szy = *szyp;
-                nhx += szx;
-                nhy += szy;
-                V nszx = bzv * szx + chz * dzEy, nszy = bzv * szy - chz * dzEx;
-#pragma unroll
-                for (int e = 0; e < VN; ++e)
-                    if (x0 + e >= a.nx - 1) { nszx[e] = szx[e]; nszy[e] = szy[e]; }
-                if (lane_on) { *szxp = nszx; *szyp = nszy; }
-            }
-            // cells i >= nx-1 keep their value (never updated by the reference)
-#pragma unroll
-            for (int e = 0; e < VN; ++e)
-                if (x0 + e >= a.nx - 1) {
-                    nhx[e] = hx[e];
-                    nhy[e] = hy[e];
-                    nhz[e] = hz[e];
-                }
-            if (lane_on) {
-                vst(nhx, reinterpret_cast<V *>(a.Hx + c));
-                vst(nhy, reinterpret_cast<V *>(a.Hy + c));
-                vst(nhz, reinterpret_cast<V *>(a.Hz + c));
+    for (int q = 0; q < VN; ++q) {
+        real x = dx[q], y = dy[q], z = dz[q];
+        for (int s = 0; s < a.nsrc; ++s) {
+            const int si = sld(a.src, 4 * s), sj = sld(a.src, 4 * s + 1), sk = sld(a.src, 4 * s + 2),
+                      sc = sld(a.src, 4 * s + 3);
+            if (sk == k && sj == j && si == x0 + q) {
+                const real v = sld(a.src_val, st * a.nsrc + s);
+                if (sc == 0) x += v;
+                else if (sc == 1) y += v;
+                else z += v;
             }
         }
+        dx[q] = x;
+        dy[q] = y;
+        dz[q] = z;
+    }
+}
+
+// D back in place, and E = M . D into set n for plain cells (fdtd.c:3404-3462).
+template <typename real>
+__device__ __forceinline__ void store_de(const YeeArgs<real> &a, const Sets<real> &s, size_t c,
+                                         const MatCoef<real> *mats, const int *mid, const int *fl, bool allplain,
+                                         typename Vec<real>::type dx, typename Vec<real>::type dy,
+                                         typename Vec<real>::type dz) {
+    typedef typename Vec<real>::type V;
+    constexpr int VN = Vec<real>::n;
+    stnt(dx, a.Dx + c);
+    stnt(dy, a.Dy + c);
+    stnt(dz, a.Dz + c);
+    if (allplain) {
+        V nex, ney, nez;
 #pragma unroll
-        for (int r = 0; r <= R; ++r) ex[r] = exn[r];
+        for (int e = 0; e < VN; ++e) {
+            const real *M = mats[mid[e]].M;
+            nex[e] = fma(M[0], dx[e], fma(M[1], dy[e], M[2] * dz[e]));
+            ney[e] = fma(M[3], dx[e], fma(M[4], dy[e], M[5] * dz[e]));
+            nez[e] = fma(M[6], dx[e], fma(M[7], dy[e], M[8] * dz[e]));
+        }
+        stnt(nex, s.En[0] + c);
+        stnt(ney, s.En[1] + c);
+        stnt(nez, s.En[2] + c);
+    } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) ey[r] = eyn[r];
+        for (int e = 0; e < VN; ++e) {
+            if (fl[e] & MF_SPECIAL) continue;
+            const real *M = mats[mid[e]].M;
+            s.En[0][c + e] = fma(M[0], dx[e], fma(M[1], dy[e], M[2] * dz[e]));
+            s.En[1][c + e] = fma(M[3], dx[e], fma(M[4], dy[e], M[5] * dz[e]));
+            s.En[2][c + e] = fma(M[6], dx[e], fma(M[7], dy[e], M[8] * dz[e]));
+        }
+    }
+}
+
+// Material ids of the lane's cells and their flags.
+template <typename real>
+__device__ __forceinline__ void cell_mats(const YeeArgs<real> &a, size_t c, const MatCoef<real> *mats, int *mid,
+                                          int *fl, bool &anyloss, bool &allplain) {
+    typedef typename Vec<real>::ids IDS;
+    constexpr int VN = Vec<real>::n;
+    const IDS idw = *reinterpret_cast<const IDS *>(a.medium + c);
+    anyloss = false;
+    allplain = true;
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+        mid[e] = (idw >> (8 * e)) & 0xff;
+        fl[e] = mats[mid[e]].flags;
+        anyloss |= (fl[e] & MF_LOSS) != 0;
+        allplain &= !(fl[e] & MF_SPECIAL);
+    }
+}
+
+// Tile t of the grid: x segment bx (64 lane vectors), row j, z chunk bz.
+struct TileOf {
+    int bx, j, bz;
+};
+template <typename real>
+__device__ __forceinline__ TileOf tile_of(const YeeArgs<real> &a, int t) {
+    TileOf r;
+    r.bx = t % a.nbx;
+    const int rest = t / a.nbx;
+    r.j = uniform((rest % a.nby) * kBlockY + threadIdx.y);
+    r.bz = rest / a.nby;
+    return r;
+}
+
+// Fused plan: H, then D and E, plane by plane, for tile t of one domain.
+template <typename real>
+__device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, const int t, const MatCoef<real> *mats) {
+    typedef typename Vec<real>::type V;
+    constexpr int VN = Vec<real>::n;
+    const TileOf tl = tile_of(a, t);
+    const int j = tl.j;
+    const int lane = threadIdx.x;
+    const int x0 = (tl.bx * 64 + lane) * VN;
+    if (j >= a.ny || uniform(tl.bx * 64 * VN) >= a.nx) return;
+    const bool store = x0 < a.nx;  // the lane holds cells (lanes past the row end still load and shuffle)
+    const int k0 = uniform(a.kbase + tl.bz * a.kchunk);
+    const int k1 = min(k0 + a.kchunk, a.kend);
+    const size_t nxp = a.nxp;
+    const size_t plane = nxp * a.ny;
+    const int64_t st = sld(a.step, 0);
+    const Sets<real> s = sets(a, st);
+    const LaneX<real> lx = lane_x(a, x0);
+    const bool hrow = j < a.ny - 1;              // H loop row
+    const bool drow = j >= 1 && j <= a.ny - 2;   // D loop row
+    const AxC<real> yc = axc(a.ay, j);
+    const AxC<real> yh = axc(a.ay, drow ? j - 1 : j);
+    const bool edge = lane == 63 && x0 + VN < a.nxp;  // loads the cell after its vector
+    const bool xhalo = lane == 0 && x0 >= VN;         // recomputes the new H of cell x0-1
+
+    size_t c = ((size_t)k0 * a.ny + j) * nxp + lx.xr;
+    V ex = ld<V>(s.Eo[0] + c), ey = ld<V>(s.Eo[1] + c), ez = ld<V>(s.Eo[2] + c);
+    V eyh = V(0);            // Ey of row j-1 at plane k
+    V hxk = V(0), hyk = V(0);  // new Hx, Hy at plane k-1 (zero below k = 0)
+    if (drow) {
+        eyh = ld<V>(s.Eo[1] + c - nxp);
+        if (k0 > 0) {  // new Hx, Hy of (j, k0-1)
+            const size_t m = c - plane;
+            const V exm = ld<V>(s.Eo[0] + m), eym = ld<V>(s.Eo[1] + m), ezm = ld<V>(s.Eo[2] + m);
+            const V ezmj = ld<V>(s.Eo[2] + m + nxp);
+            real n = shfl_down1(ezm[0]);
+            if (edge) n = s.Eo[2][m + VN];
+            V hz0 = V(0);
+            hxk = ld<V>(s.Ho[0] + m);
+            hyk = ld<V>(s.Ho[1] + m);
+            h_cells<real, HX | HY, false>(a, s, lx, yc, axc(a.az, k0 - 1), j, k0 - 1, false, exm, eym, ezm, ex, ey,
+                                          V(0), ezmj, V(0), next_x<V>(ezm, n), hxk, hyk, hz0);
+        }
+    }
+    for (int k = k0; k < k1; ++k, c += plane) {
+        const AxC<real> zc = axc(a.az, k);
+        const bool top = k + 1 >= a.nz;  // E above the top plane is the zero halo
+        V exk = V(0), eyk = V(0), ezk = V(0);
+        if (!top) {
+            exk = ld<V>(s.Eo[0] + c + plane);
+            eyk = ld<V>(s.Eo[1] + c + plane);
+            ezk = ld<V>(s.Eo[2] + c + plane);
+        }
+        V hx = ld<V>(s.Ho[0] + c), hy = ld<V>(s.Ho[1] + c), hz = ld<V>(s.Ho[2] + c);
+        V dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+        real ezn = shfl_down1(ez[0]), eyn = shfl_down1(ey[0]);
+        if (edge) {
+            ezn = s.Eo[2][c + VN];
+            eyn = s.Eo[1][c + VN];
+        }
+        if (hrow) {
+            const V exj = ld<V>(s.Eo[0] + c + nxp), ezj = ld<V>(s.Eo[2] + c + nxp);
+            h_cells<real, HX | HY | HZ, true>(a, s, lx, yc, zc, j, k, store, ex, ey, ez, exk, eyk, exj, ezj,
+                                              next_x<V>(ey, eyn), next_x<V>(ez, ezn), hx, hy, hz);
+            if (store) {
+                stnt(hx, s.Hn[0] + c);
+                stnt(hy, s.Hn[1] + c);
+                stnt(hz, s.Hn[2] + c);
+            }
+        }
+        int mid[VN], fl[VN];
+        bool anyloss, allplain;
+        cell_mats(a, c, mats, mid, fl, anyloss, allplain);
+        V eyhk = V(0);
+        if (drow) {
+            // new Hx, Hz of row j-1
+            const size_t m = c - nxp;
+            const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
+            if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
+            real n = shfl_down1(eyh[0]);
+            if (edge) n = s.Eo[1][m + VN];
+            V hxj = ld<V>(s.Ho[0] + m), hyj = V(0), hzj = ld<V>(s.Ho[2] + m);
+            h_cells<real, HX | HZ, false>(a, s, lx, yh, zc, j - 1, k, false, exh, eyh, ezh, V(0), eyhk, ex, ez,
+                                          next_x<V>(eyh, n), V(0), hxj, hyj, hzj);
+            // new Hy, Hz of the i-1 neighbours: the previous lane's last cell;
+            // lane 0 recomputes the cell before its vector
+            real hyp = shfl_up1(hy[VN - 1]), hzp = shfl_up1(hz[VN - 1]);
+            if (xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyp, hzp);
+            d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
+                    prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
+            hxk = hx;
+            hyk = hy;
+        }
+        inject(a, x0, j, k, st, dx, dy, dz);
+        if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
+        ex = exk;
+        ey = eyk;
+        ez = ezk;
+        eyh = eyhk;
+    }
+}
+
+// Split plan, first kernel: new H of tile t (one row per wavefront).
+template <typename real>
+__device__ __forceinline__ void h_tile(const YeeArgs<real> &a, const int t) {
+    typedef typename Vec<real>::type V;
+    constexpr int VN = Vec<real>::n;
+    const TileOf tl = tile_of(a, t);
+    const int j = tl.j;
+    const int lane = threadIdx.x;
+    const int x0 = (tl.bx * 64 + lane) * VN;
+    if (j >= a.ny - 1 || uniform(tl.bx * 64 * VN) >= a.nx - 1) return;
+    const bool store = x0 < a.nx - 1;
+    const int k0 = uniform(a.kbase + tl.bz * a.kchunk);
+    const int k1 = min(k0 + a.kchunk, a.kend);
+    const size_t nxp = a.nxp;
+    const size_t plane = nxp * a.ny;
+    const Sets<real> s = sets(a, sld(a.step, 0));
+    const LaneX<real> lx = lane_x(a, x0);
+    const AxC<real> yc = axc(a.ay, j);
+    const bool edge = lane == 63 && x0 + VN < a.nxp;
+    size_t c = ((size_t)k0 * a.ny + j) * nxp + lx.xr;
+    V ex = ld<V>(s.Eo[0] + c), ey = ld<V>(s.Eo[1] + c), ez = ld<V>(s.Eo[2] + c);
+    for (int k = k0; k < k1; ++k, c += plane) {
+        const AxC<real> zc = axc(a.az, k);
+        V exk = V(0), eyk = V(0), ezk = V(0);
+        if (k + 1 < a.nz) {
+            exk = ld<V>(s.Eo[0] + c + plane);
+            eyk = ld<V>(s.Eo[1] + c + plane);
+            ezk = ld<V>(s.Eo[2] + c + plane);
+        }
+        V hx = ldnt<V>(s.Ho[0] + c), hy = ldnt<V>(s.Ho[1] + c), hz = ldnt<V>(s.Ho[2] + c);
+        real ezn = shfl_down1(ez[0]), eyn = shfl_down1(ey[0]);
+        if (edge) {
+            ezn = s.Eo[2][c + VN];
+            eyn = s.Eo[1][c + VN];
+        }
+        const V exj = ld<V>(s.Eo[0] + c + nxp), ezj = ld<V>(s.Eo[2] + c + nxp);
+        h_cells<real, HX | HY | HZ, true>(a, s, lx, yc, zc, j, k, store, ex, ey, ez, exk, eyk, exj, ezj,
+                                          next_x<V>(ey, eyn), next_x<V>(ez, ezn), hx, hy, hz);
+        if (store) {
+            stnt(hx, s.Hn[0] + c);
+            stnt(hy, s.Hn[1] + c);
+            stnt(hz, s.Hn[2] + c);
+        }
+        ex = exk;
+        ey = eyk;
+        ez = ezk;
+    }
+}
+
+// Split plan, second kernel: new D and E of plain cells of tile t, from the
+// new H that h_update wrote.
+template <typename real>
+__device__ __forceinline__ void de_tile(const YeeArgs<real> &a, const int t, const MatCoef<real> *mats) {
+    typedef typename Vec<real>::type V;
+    constexpr int VN = Vec<real>::n;
+    const TileOf tl = tile_of(a, t);
+    const int j = tl.j;
+    const int lane = threadIdx.x;
+    const int x0 = (tl.bx * 64 + lane) * VN;
+    if (j >= a.ny || uniform(tl.bx * 64 * VN) >= a.nx) return;
+    const bool store = x0 < a.nx;
+    const int k0 = uniform(a.kbase + tl.bz * a.kchunk);
+    const int k1 = min(k0 + a.kchunk, a.kend);
+    const size_t nxp = a.nxp;
+    const size_t plane = nxp * a.ny;
+    const int64_t st = sld(a.step, 0);
+    const Sets<real> s = sets(a, st);
+    const real *const *Hn = s.Hn;
+    const LaneX<real> lx = lane_x(a, x0);
+    const bool drow = j >= 1 && j <= a.ny - 2;
+    const AxC<real> yc = axc(a.ay, j);
+    const bool edge = lane == 0 && x0 >= 1;  // reads the new H of the cell before its vector
+    size_t c = ((size_t)k0 * a.ny + j) * nxp + lx.xr;
+    V hxk = V(0), hyk = V(0);
+    if (drow && k0 > 0) {
+        hxk = ld<V>(Hn[0] + c - plane);
+        hyk = ld<V>(Hn[1] + c - plane);
+    }
+    for (int k = k0; k < k1; ++k, c += plane) {
+        const AxC<real> zc = axc(a.az, k);
+        V dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+        int mid[VN], fl[VN];
+        bool anyloss, allplain;
+        cell_mats(a, c, mats, mid, fl, anyloss, allplain);
+        if (drow) {
+            const V hx = ld<V>(Hn[0] + c), hy = ld<V>(Hn[1] + c), hz = ld<V>(Hn[2] + c);
+            const V hxj = ld<V>(Hn[0] + c - nxp), hzj = ld<V>(Hn[2] + c - nxp);
+            real hyp = shfl_up1(hy[VN - 1]), hzp = shfl_up1(hz[VN - 1]);
+            if (edge) {
+                hyp = Hn[1][c - 1];
+                hzp = Hn[2][c - 1];
+            }
+            V ex = V(0), ey = V(0), ez = V(0);
+            if (anyloss) {
+                ex = ld<V>(s.Eo[0] + c);
+                ey = ld<V>(s.Eo[1] + c);
+                ez = ld<V>(s.Eo[2] + c);
+            }
+            d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
+                    prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
+            hxk = hx;
+            hyk = hy;
+        }
+        inject(a, x0, j, k, st, dx, dy, dz);
+        if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
     }
 }
 
@@ -260,340 +779,103 @@ __device__ __forceinline__ MatCoef<real> *stage_mats(const YeeArgs<real> &a) {
     return reinterpret_cast<MatCoef<real> *>(lds_raw);
 }
 
-// D update and E of plain cells for tile t of one domain; mats is the LDS
-// copy of the domain's material table.
-template <typename real, int R>
-__device__ __forceinline__ void de_tile(const YeeArgs<real> &a, const int t, const MatCoef<real> *mats) {
-    constexpr int VN = Vec<real>::n;
-    typedef typename Vec<real>::type V;
-    typedef typename Vec<real>::ids IDS;
-    const int bx = t % a.nbx;
-    const int rest = t / a.nbx;
-    const int by = rest % a.nby;
-    const int bz = rest / a.nby;
-    const int lane = threadIdx.x;
-    const int x0 = (bx * 64 + lane) * VN;
-    const int j0 = uniform((by * kBlockY + threadIdx.y) * R);
-    const int nr = min(R, a.ny - j0);  // E/D rows of this wavefront
-    if (nr <= 0 || uniform(bx * 64 * VN) >= a.nx) return;
-    const bool lane_on = x0 < a.nx;
-    const int xr = x0 < a.nxp ? x0 : 0;
-    const int k0 = uniform(a.kbase + bz * a.kchunk);
-    const int k1 = min(k0 + a.kchunk, a.kend);
-    const size_t nxp = a.nxp;
-    const size_t plane = nxp * a.ny;
-    const int L2 = 2 * a.L;
-    const bool edge = lane == 0 && x0 >= 1;  // needs the cell before its vector from memory
-
-    bool inx[VN];
-    int px[VN];
-#pragma unroll
-    for (int e = 0; e < VN; ++e) {
-        const int i = x0 + e;
-        inx[e] = i >= 1 && i <= a.nx - 2;
-        px[e] = a.ax.slot[xr + e];
-    }
-    const V kdx = *reinterpret_cast<const V *>(a.ax.kd + xr);
-    bool anypx = false;
-#pragma unroll
-    for (int e = 0; e < VN; ++e) anypx |= px[e] >= 0 && inx[e];
-    // vector psi path as in h_tile (FP32)
-    bool vx = sizeof(real) == 4 && L2 % VN == 0 && px[0] >= 0 && px[0] % VN == 0;
-#pragma unroll
-    for (int e = 1; e < VN; ++e) vx = vx && px[e] == px[0] + e;
-    bool iny[R];
-    real kdy[R], byr[R], cdr[R];
-    int py[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int j = j0 + r;
-        iny[r] = j >= 1 && j <= a.ny - 2 && r < nr;
-        const int jc = min(j, a.ny - 1);
-        kdy[r] = sld(a.ay.kd, jc);
-        py[r] = sld(a.ay.slot, jc);
-        byr[r] = sld(a.ay.b, jc);
-        cdr[r] = sld(a.ay.cd, jc);
-    }
-    const bool halo = j0 >= 1;
-
-    const int64_t st = a.nsrc > 0 ? sld(a.step, 0) : 0;
-
-    size_t c0 = ((size_t)k0 * a.ny + j0) * nxp + xr;
-    V hxm[R], hym[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        hxm[r] = V(0);
-        hym[r] = V(0);
-        if (r < nr && k0 > 0) {
-            hxm[r] = *reinterpret_cast<const V *>(a.Hx + c0 - plane + r * nxp);
-            hym[r] = *reinterpret_cast<const V *>(a.Hy + c0 - plane + r * nxp);
-        }
-    }
-    for (int k = k0; k < k1; ++k, c0 += plane) {
-        V hx[R + 1], hz[R + 1], hy[R];
-        hx[0] = V(0);
-        hz[0] = V(0);
-        if (halo) {
-            hx[0] = *reinterpret_cast<const V *>(a.Hx + c0 - nxp);
-            hz[0] = *reinterpret_cast<const V *>(a.Hz + c0 - nxp);
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (r < nr) {
-                hx[r + 1] = *reinterpret_cast<const V *>(a.Hx + c0 + r * nxp);
-                hz[r + 1] = *reinterpret_cast<const V *>(a.Hz + c0 + r * nxp);
-                hy[r] = *reinterpret_cast<const V *>(a.Hy + c0 + r * nxp);
-            }
-        const real kdz = sld(a.az.kd, k);
-        const int pz = sld(a.az.slot, k);
-        const real bzv = sld(a.az.b, k), cdz = sld(a.az.cd, k);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r >= nr) continue;
-            const size_t c = c0 + r * nxp;
-            const int j = j0 + r;
-            V dx = vld(reinterpret_cast<const V *>(a.Dx + c));
-            V dy = vld(reinterpret_cast<const V *>(a.Dy + c));
-            V dz = vld(reinterpret_cast<const V *>(a.Dz + c));
-            const IDS idw = *reinterpret_cast<const IDS *>(a.medium + c);
-            int mid[VN], fl[VN], anyloss = 0, allplain = 1;
-#pragma unroll
-            for (int e = 0; e < VN; ++e) {
-                mid[e] = (idw >> (8 * e)) & 0xff;
-                fl[e] = mats[mid[e]].flags;
-                anyloss |= fl[e] & MF_LOSS;
-                allplain &= !(fl[e] & MF_SPECIAL);
-            }
-            // x neighbours at i-1: element e-1, or the last element of the previous lane
-            real hy_p = shfl_up1(hy[r][VN - 1]), hz_p = shfl_up1(hz[r + 1][VN - 1]);
-            if (edge) {
-                hy_p = a.Hy[c - 1];
-                hz_p = a.Hz[c - 1];
-            }
-            V hyim, hzim;
-            hyim[0] = hy_p;
-            hzim[0] = hz_p;
-#pragma unroll
-            for (int e = 1; e < VN; ++e) {
-                hyim[e] = hy[r][e - 1];
-                hzim[e] = hz[r + 1][e - 1];
-            }
-            if (iny[r]) {
-                const V dyHz = hz[r + 1] - hz[r], dzHy = hy[r] - hym[r];
-                const V dzHx = hx[r + 1] - hxm[r], dxHz = hz[r + 1] - hzim;
-                const V dxHy = hy[r] - hyim, dyHx = hx[r + 1] - hx[r];
-                V ndx = dx + kdy[r] * dyHz - kdz * dzHy;
-                V ndy = dy + kdz * dzHx - kdx * dxHz;
-                V ndz = dz + kdx * dxHy - kdy[r] * dyHx;
-                if (vx) {
-                    V *sxyp = reinterpret_cast<V *>(a.pDxy + ((size_t)k * a.ny + j) * L2 + px[0]);
-                    V *sxzp = reinterpret_cast<V *>(a.pDxz + ((size_t)k * a.ny + j) * L2 + px[0]);
-                    const V sxy = *sxyp, sxz = *sxzp;
-                    const V bxv = *reinterpret_cast<const V *>(a.ax.b + xr);
-                    const V cdxv = *reinterpret_cast<const V *>(a.ax.cd + xr);
-                    ndy += sxy;
-                    ndz += sxz;
-                    V nsxy = bxv * sxy - cdxv * dxHz, nsxz = bxv * sxz + cdxv * dxHy;
-#pragma unroll
-                    for (int e = 0; e < VN; ++e)
-                        if (!inx[e]) { nsxy[e] = sxy[e]; nsxz[e] = sxz[e]; }
-                    if (lane_on) { *sxyp = nsxy; *sxzp = nsxz; }
-                } else if (anypx) {
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) {
-                        if (px[e] < 0 || !inx[e]) continue;
-                        const int i = xr + e;
-                        const size_t q = ((size_t)k * a.ny + j) * L2 + px[e];
-                        const real sxy = a.pDxy[q], sxz = a.pDxz[q];
-                        ndy[e] += sxy;
-                        ndz[e] += sxz;
-                        a.pDxy[q] = a.ax.b[i] * sxy - a.ax.cd[i] * dxHz[e];
-                        a.pDxz[q] = a.ax.b[i] * sxz + a.ax.cd[i] * dxHy[e];
-                    }
-                }
-                if (py[r] >= 0) {
-                    V *syxp = reinterpret_cast<V *>(a.pDyx + ((size_t)k * L2 + py[r]) * nxp + xr);
-                    V *syzp = reinterpret_cast<V *>(a.pDyz + ((size_t)k * L2 + py[r]) * nxp + xr);
-                    const real byv = byr[r], cdy = cdr[r];
-                    const V syx = *syxp, syz = *syzp;
-                    ndx += syx;
-                    ndz += syz;
-                    // psi of cells outside the D loop (i = 0, nx-1) is never advanced
-                    V nsyx = byv * syx + cdy * dyHz, nsyz = byv * syz - cdy * dyHx;
-#pragma unroll
-                    for (int e = 0; e < VN; ++e)
-                        if (!inx[e]) { nsyx[e] = syx[e]; nsyz[e] = syz[e]; }
-                    if (lane_on) { *syxp = nsyx; *syzp = nsyz; }
-                }
-                if (pz >= 0) {
-                    V *szxp = reinterpret_cast<V *>(a.pDzx + ((size_t)pz * a.ny + j) * nxp + xr);
-                    V *szyp = reinterpret_cast<V *>(a.pDzy + ((size_t)pz * a.ny + j) * nxp + xr);
-                    const V szx = *szxp, szy = *szyp;
-                    ndx += szx;
-                    ndy += szy;
-                    V nszx = bzv * szx - cdz * dzHy, nszy = bzv * szy + cdz * dzHx;
-#pragma unroll
-                    for (int e = 0; e < VN; ++e)
-                        if (!inx[e]) { nszx[e] = szx[e]; nszy[e] = szy[e]; }
-                    if (lane_on) { *szxp = nszx; *szyp = nszy; }
-                }
-                if (anyloss) {
-                    const V ex = *reinterpret_cast<const V *>(a.Ex + c);
-                    const V ey = *reinterpret_cast<const V *>(a.Ey + c);
-                    const V ez = *reinterpret_cast<const V *>(a.Ez + c);
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) {
-                        if (!(fl[e] & MF_LOSS)) continue;
-                        const real *Lk = mats[mid[e]].Lk;
-                        ndx[e] -= Lk[0] * ex[e] + Lk[1] * ey[e] + Lk[2] * ez[e];
-                        ndy[e] -= Lk[3] * ex[e] + Lk[4] * ey[e] + Lk[5] * ez[e];
-                        ndz[e] -= Lk[6] * ex[e] + Lk[7] * ey[e] + Lk[8] * ez[e];
-                    }
-                }
-#pragma unroll
-                for (int e = 0; e < VN; ++e)
-                    if (inx[e]) {
-                        dx[e] = ndx[e];
-                        dy[e] = ndy[e];
-                        dz[e] = ndz[e];
-                    }
-            }
-            for (int s = 0; s < a.nsrc; ++s) {
-                const int si = sld(a.src, 4 * s), sj = sld(a.src, 4 * s + 1), sk = sld(a.src, 4 * s + 2),
-                          sc = sld(a.src, 4 * s + 3);
-                if (sk == k && sj == j) {
-                    const int e = si - x0;
-                    if (e >= 0 && e < VN) {
-                        const real v = sld(a.src_val, st * a.nsrc + s);
-#pragma unroll
-                        for (int q = 0; q < VN; ++q)
-                            if (q == e) {
-                                if (sc == 0) dx[q] += v;
-                                else if (sc == 1) dy[q] += v;
-                                else dz[q] += v;
-                            }
-                    }
-                }
-            }
-            if (lane_on) {
-                vst(dx, reinterpret_cast<V *>(a.Dx + c));
-                vst(dy, reinterpret_cast<V *>(a.Dy + c));
-                vst(dz, reinterpret_cast<V *>(a.Dz + c));
-                if (allplain) {
-                    V nex, ney, nez;
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) {
-                        const real *M = mats[mid[e]].M;
-                        nex[e] = M[0] * dx[e] + M[1] * dy[e] + M[2] * dz[e];
-                        ney[e] = M[3] * dx[e] + M[4] * dy[e] + M[5] * dz[e];
-                        nez[e] = M[6] * dx[e] + M[7] * dy[e] + M[8] * dz[e];
-                    }
-                    vst(nex, reinterpret_cast<V *>(a.Ex + c));
-                    vst(ney, reinterpret_cast<V *>(a.Ey + c));
-                    vst(nez, reinterpret_cast<V *>(a.Ez + c));
-                } else {
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) {
-                        if (fl[e] & MF_SPECIAL) continue;
-                        const real *M = mats[mid[e]].M;
-                        a.Ex[c + e] = M[0] * dx[e] + M[1] * dy[e] + M[2] * dz[e];
-                        a.Ey[c + e] = M[3] * dx[e] + M[4] * dy[e] + M[5] * dz[e];
-                        a.Ez[c + e] = M[6] * dx[e] + M[7] * dy[e] + M[8] * dz[e];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            hxm[r] = hx[r + 1];
-            hym[r] = hy[r];
-        }
-    }
+template <typename real>
+__global__ __launch_bounds__(256) void yee_update(YeeArgs<real> a) {
+    const MatCoef<real> *mats = stage_mats(a);
+    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
+    if (t < 0) return;
+    yee_tile<real>(a, t, mats);
 }
 
-template <typename real, int R>
+template <typename real>
 __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
     const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
     if (t < 0) return;
-    h_tile<real, R>(a, t);
+    h_tile<real>(a, t);
 }
 
-template <typename real, int R>
+template <typename real>
 __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a) {
     const MatCoef<real> *mats = stage_mats(a);
     const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
     if (t < 0) return;
-    de_tile<real, R>(a, t, mats);
+    de_tile<real>(a, t, mats);
 }
 
 // Batched launches: the tiles of every domain of a batch in one grid.  The
 // XCD renumbering runs over the concatenated tiles.
 
-template <typename real, int R>
+template <typename real>
+__global__ __launch_bounds__(256) void yee_update_batch(BatchLaunch<YeeArgs<real>> b) {
+    const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
+    if (g < 0) return;  // whole work-group
+    const int rid = (int)sld(b.map, g);
+    const YeeArgs<real> a = batch_args(b, rid);
+    const MatCoef<real> *mats = stage_mats(a);
+    yee_tile<real>(a, g - sld(b.start, rid), mats);
+}
+
+template <typename real>
 __global__ __launch_bounds__(256) void h_update_batch(BatchLaunch<YeeArgs<real>> b) {
     const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
     if (g < 0) return;
     const int rid = (int)sld(b.map, g);
     const YeeArgs<real> a = batch_args(b, rid);
-    h_tile<real, R>(a, g - sld(b.start, rid));
+    h_tile<real>(a, g - sld(b.start, rid));
 }
 
-template <typename real, int R>
+template <typename real>
 __global__ __launch_bounds__(256) void de_update_batch(BatchLaunch<YeeArgs<real>> b) {
     const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
     if (g < 0) return;  // whole work-group
     const int rid = (int)sld(b.map, g);
     const YeeArgs<real> a = batch_args(b, rid);
     const MatCoef<real> *mats = stage_mats(a);
-    de_tile<real, R>(a, g - sld(b.start, rid), mats);
+    de_tile<real>(a, g - sld(b.start, rid), mats);
 }
 
-template <typename real, int R>
-static void launch_h_r(const YeeArgs<real> &a, hipStream_t s) {
-    hipLaunchKernelGGL((h_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), 0, s, a);
-}
-
-template <typename real, int R>
-static void launch_de_r(const YeeArgs<real> &a, hipStream_t s) {
+template <typename real>
+void launch_yee(const YeeArgs<real> &a, hipStream_t s) {
     const size_t lds = (size_t)a.nmat * sizeof(MatCoef<real>);
-    hipLaunchKernelGGL((de_update<real, R>), dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a);
+    hipLaunchKernelGGL(yee_update<real>, dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a);
 }
 
 template <typename real>
 void launch_h(const YeeArgs<real> &a, hipStream_t s) {
-    switch (a.rows) {
-        case 1: launch_h_r<real, 1>(a, s); break;
-        case 2: launch_h_r<real, 2>(a, s); break;
-        default: launch_h_r<real, 4>(a, s); break;
-    }
+    hipLaunchKernelGGL(h_update<real>, dim3(a.nblocks_pad), dim3(64, kBlockY), 0, s, a);
 }
 
 template <typename real>
 void launch_de(const YeeArgs<real> &a, hipStream_t s) {
-    switch (a.rows) {
-        case 1: launch_de_r<real, 1>(a, s); break;
-        case 2: launch_de_r<real, 2>(a, s); break;
-        default: launch_de_r<real, 4>(a, s); break;
-    }
+    const size_t lds = (size_t)a.nmat * sizeof(MatCoef<real>);
+    hipLaunchKernelGGL(de_update<real>, dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a);
 }
 
-// Batched H and D/E over the domains of a batch (one y row per wavefront).
+template <typename real>
+void launch_yee_batch(const BatchLaunch<YeeArgs<real>> &b, int max_nmat, hipStream_t s) {
+    const size_t lds = (size_t)max_nmat * sizeof(MatCoef<real>);
+    if (b.total > 0) hipLaunchKernelGGL(yee_update_batch<real>, dim3(b.total_pad), dim3(64, kBlockY), lds, s, b);
+}
+
 template <typename real>
 void launch_h_batch(const BatchLaunch<YeeArgs<real>> &b, hipStream_t s) {
-    if (b.total > 0) hipLaunchKernelGGL((h_update_batch<real, 1>), dim3(b.total_pad), dim3(64, kBlockY), 0, s, b);
+    if (b.total > 0) hipLaunchKernelGGL(h_update_batch<real>, dim3(b.total_pad), dim3(64, kBlockY), 0, s, b);
 }
 
 template <typename real>
 void launch_de_batch(const BatchLaunch<YeeArgs<real>> &b, int max_nmat, hipStream_t s) {
     const size_t lds = (size_t)max_nmat * sizeof(MatCoef<real>);
-    if (b.total > 0)
-        hipLaunchKernelGGL((de_update_batch<real, 1>), dim3(b.total_pad), dim3(64, kBlockY), lds, s, b);
+    if (b.total > 0) hipLaunchKernelGGL(de_update_batch<real>, dim3(b.total_pad), dim3(64, kBlockY), lds, s, b);
 }
 
+template void launch_yee<double>(const YeeArgs<double> &, hipStream_t);
+template void launch_yee<float>(const YeeArgs<float> &, hipStream_t);
 template void launch_h<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_h<float>(const YeeArgs<float> &, hipStream_t);
 template void launch_de<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_de<float>(const YeeArgs<float> &, hipStream_t);
+template void launch_yee_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);
+template void launch_yee_batch<float>(const BatchLaunch<YeeArgs<float>> &, int, hipStream_t);
 template void launch_h_batch<double>(const BatchLaunch<YeeArgs<double>> &, hipStream_t);
 template void launch_h_batch<float>(const BatchLaunch<YeeArgs<float>> &, hipStream_t);
 template void launch_de_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);

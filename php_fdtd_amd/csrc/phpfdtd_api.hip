@@ -26,8 +26,10 @@
 #include "phpfdtd_internal.h"
 
 namespace phpfdtd {
+template <typename real> void launch_yee(const YeeArgs<real> &, hipStream_t);
 template <typename real> void launch_h(const YeeArgs<real> &, hipStream_t);
 template <typename real> void launch_de(const YeeArgs<real> &, hipStream_t);
+template <typename real> void launch_yee_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
 template <typename real> void launch_h_batch(const BatchLaunch<YeeArgs<real>> &, hipStream_t);
 template <typename real> void launch_de_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
 template <typename real> void launch_slot_batch(const BatchLaunch<SlotArgs<real>> &, int, hipStream_t);
@@ -87,11 +89,9 @@ struct Special {
     SlotArgs<float> sa32{};
 };
 
-// Stages of one time step, in issue order.  With overlap (PHPFDTD_OVERLAP=1),
-// the H ranges that read no lattice-cell E (K_HA) of step n+1 run on a second
-// stream beside the lattice kernels (K_ION) of step n; every other order, and
-// the default plan, is sequential.
-enum Stage { K_HA, K_HB, K_DE, K_PRE, K_ION, K_POST };
+// Stages of one time step, in issue order: the Yee kernel(s), the ADE slot
+// kernels, samples of D and H, the lattice kernels, the epilogue.
+enum Stage { K_YEE, K_DE, K_PRE, K_ION, K_POST };
 
 struct Kernel {
     std::string name;
@@ -106,7 +106,8 @@ struct Layout {
     int nxp = 0;
     size_t ncellp = 0;
     size_t field = 0, psi_x = 0, psi_y = 0, psi_z = 0, medium = 0, small = 0, slots = 0, ring = 0;
-    size_t total() const { return 9 * field + 4 * (psi_x + psi_y + psi_z) + medium + small + slots + ring; }
+    // E, H in two sets and D (15 field arrays); H psi in two sets and D psi
+    size_t total() const { return 15 * field + 6 * (psi_x + psi_y + psi_z) + medium + small + slots + ring; }
 };
 
 }  // namespace
@@ -122,8 +123,12 @@ struct phpfdtd_ctx {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     size_t bytes = 0;
-    void *F[9] = {};
-    void *psi[12] = {};  // Hxy Hxz Dxy Dxz | Hyx Hyz Dyx Dyz | Hzx Hzy Dzx Dzy
+    // field arrays of E/H sets 0 and 1 (Ex..Hz; the D entries of set 1 are
+    // those of set 0), and CPML psi slabs likewise (Hxy Hxz Dxy Dxz | Hyx Hyz
+    // Dyx Dyz | Hzx Hzy Dzx Dzy; the D slabs single).  After s steps set s & 1
+    // is current.
+    void *F[2][9] = {};
+    void *psi[2][12] = {};
     uint8_t *d_medium = nullptr;
     std::vector<uint8_t> medium;  // host copy, device layout
     int nmat = 0;
@@ -131,10 +136,12 @@ struct phpfdtd_ctx {
     void *d_mats = nullptr;
     void *d_axis[3][5] = {};
     int *d_slot[3] = {};
+    void *d_axrow[3] = {};  // y and z coefficients packed per row / plane (AxRow)
     std::vector<Special> specials;
     int special_of[kMaxMat];
     int64_t lossy_interior = 0;  // interior cells of materials with eps_im loss
-    int yee_rows = 1, yee_kchunk = 16;  // Yee tile: y rows per wavefront, z planes per chunk
+    bool fused = true;     // one-pass Yee kernel (yee_update), else h_update + de_update
+    int yee_kchunk = 16;   // z planes per Yee tile
     // sources
     int nsrc = 0;
     int *d_src = nullptr;  // [kMaxSrc][4] (i, j, k, component)
@@ -165,10 +172,6 @@ struct phpfdtd_ctx {
     hipEvent_t ev = nullptr;  // marks this stream's latest work for the batch stream
     // plan and graphs
     std::vector<Kernel> plan;
-    bool allow_overlap = false;           // PHPFDTD_OVERLAP=1 turns the overlap on
-    bool overlap = false;                 // plan has K_HA ranges to overlap with K_ION
-    hipStream_t stream2 = nullptr;        // second stream of the overlapped graphs
-    std::vector<hipEvent_t> cap_events;   // fork/join events of the captured graphs
     hipGraphExec_t g1 = nullptr, g16 = nullptr;
     bool dirty = true;
     bool use_graph = true;
@@ -180,9 +183,7 @@ struct phpfdtd_ctx {
         if (g1) (void)hipGraphExecDestroy(g1);
         if (g16) (void)hipGraphExecDestroy(g16);
         if (ev) (void)hipEventDestroy(ev);
-        for (hipEvent_t e : cap_events) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
-        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -219,6 +220,9 @@ struct phpfdtd_ctx {
 
     size_t fidx(int i, int j, int k) const { return ((size_t)k * ny + j) * nxp + i; }
 
+    // component q (0..8) of the current E/H set (after host_step steps)
+    void *cur(int q) const { return F[host_step & 1][q]; }
+
     // free one allocation made by alloc (after the stream is idle)
     void release(void *p) {
         if (!p) return;
@@ -244,6 +248,7 @@ static Layout make_layout(const phpfdtd_grid &g, const std::vector<int64_t> &nsl
     l.psi_z = L2 * l.nxp * g.ny * rs;
     l.medium = l.ncellp;
     l.small = kMaxMat * sizeof(MatCoef<double>) + 3 * ((size_t)g.nx + g.ny + g.nz) * (5 * rs + 4) +
+              ((size_t)g.ny + g.nz) * sizeof(AxRow<double>) +
               (size_t)g.tsteps * kMaxSrc * rs + 1024 + 64 * 1024 + (size_t)kDiagWaves * 16;
     for (size_t m = 0; m < nslot.size(); ++m) {
         const size_t n = nslot[m];
@@ -313,50 +318,70 @@ template <typename real> static SlotArgs<real> &slot_args_of(Special &sp);
 template <> SlotArgs<double> &slot_args_of<double>(Special &sp) { return sp.sa64; }
 template <> SlotArgs<float> &slot_args_of<float>(Special &sp) { return sp.sa32; }
 
+// Yee plan switches, read at creation and whenever the plan is rebuilt (so a
+// context can be timed under several settings): PHPFDTD_YEE=split runs H and
+// D/E as two kernels (same results) instead of the one-pass kernel;
+// PHPFDTD_YEE_KCHUNK sets the z planes per tile (defaults measured on MI355X,
+// 1024x1024x640).  A batch member keeps the plan it joined with.
+static void plan_env(phpfdtd_ctx *x) {
+    if (x->batch) return;
+    const char *yp = getenv("PHPFDTD_YEE");
+    x->fused = !(yp && strcmp(yp, "split") == 0);
+    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? (x->fused ? 16 : 4) : 16;
+    if (const char *e = getenv("PHPFDTD_YEE_KCHUNK")) {
+        const int v = atoi(e);
+        if (v >= 1) x->yee_kchunk = v;
+    }
+}
+
 template <typename real>
 static void build_plan_t(phpfdtd_ctx *x) {
+    plan_env(x);
     x->plan.clear();
     const size_t w = sizeof(real);
     YeeArgs<real> a{};
-    a.Ex = (real *)x->F[0]; a.Ey = (real *)x->F[1]; a.Ez = (real *)x->F[2];
-    a.Dx = (real *)x->F[3]; a.Dy = (real *)x->F[4]; a.Dz = (real *)x->F[5];
-    a.Hx = (real *)x->F[6]; a.Hy = (real *)x->F[7]; a.Hz = (real *)x->F[8];
-    real **ps[12] = {&a.pHxy, &a.pHxz, &a.pDxy, &a.pDxz, &a.pHyx, &a.pHyz,
-                     &a.pDyx, &a.pDyz, &a.pHzx, &a.pHzy, &a.pDzx, &a.pDzy};
-    for (int q = 0; q < 12; ++q) *ps[q] = (real *)x->psi[q];
+    for (int set = 0; set < 2; ++set) {
+        for (int q = 0; q < 3; ++q) {
+            a.E[set][q] = (real *)x->F[set][q];
+            a.H[set][q] = (real *)x->F[set][6 + q];
+        }
+        const int hpsi[6] = {0, 1, 4, 5, 8, 9};  // Hxy Hxz Hyx Hyz Hzx Hzy
+        for (int q = 0; q < 6; ++q) a.pH[set][q] = (real *)x->psi[set][hpsi[q]];
+    }
+    a.Dx = (real *)x->F[0][3]; a.Dy = (real *)x->F[0][4]; a.Dz = (real *)x->F[0][5];
+    a.pDxy = (real *)x->psi[0][2]; a.pDxz = (real *)x->psi[0][3];
+    a.pDyx = (real *)x->psi[0][6]; a.pDyz = (real *)x->psi[0][7];
+    a.pDzx = (real *)x->psi[0][10]; a.pDzy = (real *)x->psi[0][11];
     a.medium = x->d_medium;
     a.mats = (const MatCoef<real> *)x->d_mats;
-    Axis<real> *axes[3] = {&a.ax, &a.ay, &a.az};
-    for (int d = 0; d < 3; ++d) {
-        axes[d]->b = (const real *)x->d_axis[d][0];
-        axes[d]->kh = (const real *)x->d_axis[d][1];
-        axes[d]->kd = (const real *)x->d_axis[d][2];
-        axes[d]->ch = (const real *)x->d_axis[d][3];
-        axes[d]->cd = (const real *)x->d_axis[d][4];
-        axes[d]->slot = x->d_slot[d];
-    }
+    a.ax.b = (const real *)x->d_axis[0][0];
+    a.ax.kh = (const real *)x->d_axis[0][1];
+    a.ax.kd = (const real *)x->d_axis[0][2];
+    a.ax.ch = (const real *)x->d_axis[0][3];
+    a.ax.cd = (const real *)x->d_axis[0][4];
+    a.ax.slot = x->d_slot[0];
+    a.ay = (const AxRow<real> *)x->d_axrow[1];
+    a.az = (const AxRow<real> *)x->d_axrow[2];
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp; a.L = x->L;
     a.nmat = x->nmat;
-    // a work-group is 4 wavefronts, each `rows` y rows x 64 lanes x one
-    // 16-byte vector of x cells, marching over a short chunk of z planes
-    // (streams of many short-lived wavefronts run closest to the HBM peak)
-    a.rows = x->yee_rows;
+    // a work-group is 4 wavefronts, each one y row x 64 lanes x one 16-byte
+    // vector of x cells, marching over a short chunk of z planes (streams of
+    // many short-lived wavefronts run closest to the HBM peak)
     const int vec = YeeVec<real>::value;
     a.nbx = (x->nx + 64 * vec - 1) / (64 * vec);
-    a.nby = (x->ny + kBlockY * a.rows - 1) / (kBlockY * a.rows);
-    int kc = std::min(x->yee_kchunk, x->nz);
-    const int nbz = (x->nz + kc - 1) / kc;
-    const int bxy = a.nbx * a.nby;
+    a.nby = (x->ny + kBlockY - 1) / kBlockY;
+    const int kc = std::min(x->yee_kchunk, x->nz);
     a.kchunk = kc;
     a.kbase = 0;
     a.kend = x->nz;
-    a.nbz = nbz;
-    a.nblocks = bxy * nbz;
+    a.nbz = (x->nz + kc - 1) / kc;
+    a.nblocks = a.nbx * a.nby * a.nbz;
     a.nblocks_pad = (a.nblocks + 7) / 8 * 8;
     a.nsrc = x->nsrc;
     a.src = x->d_src;
     a.src_val = (const real *)x->d_src_val;
     a.step = x->d_step;
+    args_of<real>(x) = a;
 
     // algorithmic bytes (compulsory traffic of each kernel's own streams)
     const double nx = x->nx, ny = x->ny, nz = x->nz, L = x->L;
@@ -370,55 +395,40 @@ static void build_plan_t(phpfdtd_ctx *x) {
     double special_cells = 0;
     for (const auto &sp : x->specials) special_cells += (double)sp.cells.size();
     const double lossy_int = (double)x->lossy_interior;
-    // H over z ranges: with a lattice material, planes [klo, khi) read E of
-    // lattice cells (plane k reads E at k and k+1); the others do not and may
-    // run beside the previous step's lattice kernels
-    int klo = x->nz, khi = 0;
-    for (const auto &sp : x->specials)
-        if (sp.pre.natom > 0 && !sp.cells.empty()) {
-            klo = std::min(klo, (int)(sp.cells.front() / ((size_t)x->nxp * x->ny)) - 1);
-            khi = std::max(khi, (int)(sp.cells.back() / ((size_t)x->nxp * x->ny)) + 1);
-        }
-    klo = std::max(klo, 0);
-    khi = std::min(khi, x->nz);
-    x->overlap = x->allow_overlap && khi > klo && (klo > 0 || khi < x->nz);
-    auto h_range = [&](int kb, int ke, int stage, const char *nm) {
-        if (ke <= kb) return;
-        YeeArgs<real> r = a;
-        r.kbase = kb;
-        r.kend = ke;
-        r.nbz = (ke - kb + kc - 1) / kc;
-        r.nblocks = bxy * r.nbz;
-        r.nblocks_pad = (r.nblocks + 7) / 8 * 8;
-        const double f = (double)(ke - kb) / nz;
-        Kernel kh;
-        kh.name = nm;
-        kh.stage = stage;
-        kh.bytes = f * (hcells * 9 * w + pml_inc(nx - 1, ny - 1, nz) * 4 * w);
-        kh.flops = f * hcells * 18;
-        kh.fn = [r](hipStream_t s) { launch_h<real>(r, s); };
-        x->plan.push_back(kh);
-    };
-    if (x->overlap) {
-        h_range(0, klo, K_HA, "h_update[lo]");
-        h_range(khi, x->nz, K_HA, "h_update[hi]");
-        h_range(klo, khi, K_HB, "h_update[lattice]");
+    const double psi_h = pml_inc(nx - 1, ny - 1, nz) * 4 * w, psi_d = pml_inc(nx - 2, ny - 2, nz) * 4 * w;
+    if (x->fused) {
+        // E read once (3w), E of plain cells written (3w), H read and written
+        // (6w), D read and written (6w), material id, psi of both halves
+        Kernel ky;
+        ky.name = "yee_update";
+        ky.stage = K_YEE;
+        ky.bytes = cells * (9 * w + 1) + (cells - special_cells) * 3 * w + hcells * 6 * w + psi_h + psi_d;
+        ky.flops = hcells * 18 + icells * 18 + cells * 15;
+        ky.fn = [a](hipStream_t s) { launch_yee<real>(a, s); };
+        x->plan.push_back(ky);
     } else {
-        h_range(0, x->nz, K_HB, "h_update");
+        Kernel kh;
+        kh.name = "h_update";
+        kh.stage = K_YEE;
+        kh.bytes = hcells * 9 * w + psi_h;
+        kh.flops = hcells * 18;
+        kh.fn = [a](hipStream_t s) { launch_h<real>(a, s); };
+        x->plan.push_back(kh);
+        Kernel kd;
+        kd.name = "de_update";
+        kd.stage = K_YEE;
+        kd.bytes = cells * (6 * w + 1) + (cells - special_cells) * 3 * w + icells * 3 * w + lossy_int * 3 * w + psi_d;
+        kd.flops = icells * 18 + cells * 15;
+        kd.fn = [a](hipStream_t s) { launch_de<real>(a, s); };
+        x->plan.push_back(kd);
     }
-    Kernel kd;
-    kd.name = "de_update";
-    kd.bytes = cells * (6 * w + 1) + (cells - special_cells) * 3 * w + icells * 3 * w + lossy_int * 3 * w +
-               pml_inc(nx - 2, ny - 2, nz) * 4 * w;
-    kd.flops = icells * 18 + cells * 15;
-    kd.fn = [a](hipStream_t s) { launch_de<real>(a, s); };
-    args_of<real>(x) = a;
-    x->plan.push_back(kd);
 
     for (auto &sp : x->specials) {
         SlotArgs<real> sa{};
-        sa.Ex = a.Ex; sa.Ey = a.Ey; sa.Ez = a.Ez;
+        for (int set = 0; set < 2; ++set)
+            for (int q = 0; q < 3; ++q) sa.E[set][q] = a.E[set][q];
         sa.Dx = a.Dx; sa.Dy = a.Dy; sa.Dz = a.Dz;
+        sa.step = x->d_step;
         sa.cells = sp.d_cells;
         sa.nslot = (int64_t)sp.cells.size();
         for (int e = 0; e < 9; ++e) {
@@ -479,7 +489,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
     se.ring_rows = x->ring_rows;
     se.mon_on = x->mon_on ? 1 : 0;
     se.mon_c = x->mon_c;
-    for (int q = 0; q < 9; ++q) se.F[q] = x->F[q];
+    for (int set = 0; set < 2; ++set)
+        for (int q = 0; q < 9; ++q) se.F[set][q] = x->F[set][q];
     se.mon_r = x->mon_r;
     se.mon_v = x->mon_v;
     se.mon_rc = x->mon_rc;
@@ -489,26 +500,11 @@ static void build_plan_t(phpfdtd_ctx *x) {
     se.dt = x->g.dt;
     se.part = 0;
     x->se = se;
-    if (x->overlap) {
-        StepEndArgs s1 = se, s2 = se;
-        s1.part = 1;
-        s2.part = 2;
-        Kernel k1, k2;
-        k1.name = "step_end[dh]";
-        k1.stage = K_PRE;
-        k1.fn = [s1](hipStream_t s) { launch_step_end<real>(s1, s); };
-        k2.name = "step_end";
-        k2.stage = K_POST;
-        k2.fn = [s2](hipStream_t s) { launch_step_end<real>(s2, s); };
-        x->plan.push_back(k1);
-        x->plan.push_back(k2);
-    } else {
-        Kernel ke;
-        ke.name = "step_end";
-        ke.stage = K_POST;
-        ke.fn = [se](hipStream_t s) { launch_step_end<real>(se, s); };
-        x->plan.push_back(ke);
-    }
+    Kernel ke;
+    ke.name = "step_end";
+    ke.stage = K_POST;
+    ke.fn = [se](hipStream_t s) { launch_step_end<real>(se, s); };
+    x->plan.push_back(ke);
     // issue order of one step
     std::stable_sort(x->plan.begin(), x->plan.end(),
                      [](const Kernel &p, const Kernel &q) { return p.stage < q.stage; });
@@ -531,38 +527,8 @@ static int rebuild(phpfdtd_ctx *x) {
         const int n = which == 0 ? 1 : 16;
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
-        if (!x->overlap) {
-            for (int t = 0; t < n; ++t)
-                for (auto &k : x->plan) k.fn(x->stream);
-        } else {
-            // step t+1's free H ranges fork onto stream2 after step t's
-            // D/H sampling and join before step t's epilogue part 2
-            while ((int)x->cap_events.size() < 2 * n) {
-                hipEvent_t e;
-                HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-                x->cap_events.push_back(e);
-            }
-            for (auto &k : x->plan)
-                if (k.stage == K_HA) k.fn(x->stream);
-            for (int t = 0; t < n; ++t) {
-                const bool fork = t + 1 < n;
-                for (auto &k : x->plan) {
-                    if (k.stage == K_HA) continue;
-                    if (k.stage == K_ION && fork && &k == &*std::find_if(x->plan.begin(), x->plan.end(),
-                                                                         [](const Kernel &q) { return q.stage == K_ION; })) {
-                        HIPCHK(hipEventRecord(x->cap_events[2 * t], x->stream));
-                        HIPCHK(hipStreamWaitEvent(x->stream2, x->cap_events[2 * t], 0));
-                        for (auto &h : x->plan)
-                            if (h.stage == K_HA) h.fn(x->stream2);
-                        HIPCHK(hipEventRecord(x->cap_events[2 * t + 1], x->stream2));
-                    }
-                    if (k.stage == K_POST && fork && &k == &*std::find_if(x->plan.begin(), x->plan.end(),
-                                                                          [](const Kernel &q) { return q.stage == K_POST; }))
-                        HIPCHK(hipStreamWaitEvent(x->stream, x->cap_events[2 * t + 1], 0));
-                    k.fn(x->stream);
-                }
-            }
-        }
+        for (int t = 0; t < n; ++t)
+            for (auto &k : x->plan) k.fn(x->stream);
         HIPCHK(hipStreamEndCapture(x->stream, &graph));
         hipGraphExec_t ex = nullptr;
         hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
@@ -590,6 +556,22 @@ static int upload_mats(phpfdtd_ctx *x) {
         mc[m].flags = (x->pre[m].loss ? MF_LOSS : 0) | (x->pre[m].special ? MF_SPECIAL : 0);
     }
     return x->upload(&x->d_mats, mc);
+}
+
+// Coefficient rows of a y or z axis, packed (AxRow).
+template <typename real>
+static int upload_rows(phpfdtd_ctx *x, const AxisHost &ax, void **out) {
+    std::vector<AxRow<real>> rows(ax.kh.size());
+    for (size_t i = 0; i < rows.size(); ++i) {
+        memset(&rows[i], 0, sizeof(rows[i]));
+        rows[i].kh = (real)ax.kh[i];
+        rows[i].kd = (real)ax.kd[i];
+        rows[i].b = (real)ax.b[i];
+        rows[i].ch = (real)ax.ch[i];
+        rows[i].cd = (real)ax.cd[i];
+        rows[i].p = ax.slot[i];
+    }
+    return x->upload(out, rows);
 }
 
 // Every int entry point runs its body through guarded(): a C++ exception
@@ -656,25 +638,7 @@ static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int 
     x->nmat = nmat;
     const char *ng = getenv("PHPFDTD_NO_GRAPH");
     x->use_graph = !(ng && ng[0] == '1');
-    // H ranges beside the lattice kernel: off by default.  Every kernel of the
-    // step streams near the HBM limit, so running two side by side gains
-    // nothing, while splitting H into three launches costs 0.3 ms (FP32 step,
-    // 1024x1024x640: 21.69 ms with one H launch, 22.08 ms overlapped)
-    const char *ov = getenv("PHPFDTD_OVERLAP");
-    x->allow_overlap = ov && ov[0] == '1';
-    // Yee tile shape, defaults measured on MI355X (1024x1024x640, same-context
-    // comparisons): FP32 1 y row x 4 planes, FP64 2 y rows x 16 planes;
-    // overridable for tuning
-    x->yee_rows = x->prec == PHPFDTD_FP32 ? 1 : 2;
-    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 4 : 16;
-    if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
-        const int v = atoi(e);
-        if (v == 1 || v == 2 || v == 4) x->yee_rows = v;
-    }
-    if (const char *e = getenv("PHPFDTD_YEE_KCHUNK")) {
-        const int v = atoi(e);
-        if (v >= 1) x->yee_kchunk = v;
-    }
+    plan_env(x);
     x->pre.resize(nmat);
     for (int m = 0; m < nmat; ++m) {
         std::string err;
@@ -683,7 +647,6 @@ static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int 
     }
     HIPCHK(hipGetDevice(&x->device));
     HIPCHK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&x->stream2, hipStreamNonBlocking));
 
     std::vector<int> natom(nmat), ade(nmat);
     for (int m = 0; m < nmat; ++m) {
@@ -698,16 +661,21 @@ static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int 
     x->ncellp = lay.ncellp;
     x->plane = (size_t)x->nxp * x->ny;
 
-    for (int q = 0; q < 9; ++q)
-        if ((rc = x->alloc(&x->F[q], lay.field))) return rc;
-    if (x->L > 0) {
-        for (int q = 0; q < 4; ++q)
-            if ((rc = x->alloc(&x->psi[q], lay.psi_x))) return rc;
-        for (int q = 4; q < 8; ++q)
-            if ((rc = x->alloc(&x->psi[q], lay.psi_y))) return rc;
-        for (int q = 8; q < 12; ++q)
-            if ((rc = x->alloc(&x->psi[q], lay.psi_z))) return rc;
-    }
+    // E and H in two sets, D in one; H psi slabs in two sets, D psi in one
+    for (int set = 0; set < 2; ++set)
+        for (int q = 0; q < 9; ++q) {
+            const bool d = q >= 3 && q < 6;
+            if (set == 1 && d) x->F[1][q] = x->F[0][q];
+            else if ((rc = x->alloc(&x->F[set][q], lay.field))) return rc;
+        }
+    if (x->L > 0)
+        for (int set = 0; set < 2; ++set)
+            for (int q = 0; q < 12; ++q) {
+                const bool d = (q & 3) >= 2;  // Dxy Dxz | Dyx Dyz | Dzx Dzy
+                if (set == 1 && d) x->psi[1][q] = x->psi[0][q];
+                else if ((rc = x->alloc(&x->psi[set][q], q < 4 ? lay.psi_x : q < 8 ? lay.psi_y : lay.psi_z)))
+                    return rc;
+            }
     // medium: file order [nx][ny][nz] -> x fastest; padding cells get id 0
     x->medium.assign(x->ncellp, 0);
     for (int i = 0; i < x->nx; ++i)
@@ -735,6 +703,11 @@ static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int 
         for (int t = 0; t < 5; ++t)
             if ((rc = x->upload_real(&x->d_axis[d][t], *tabs[t]))) return rc;
         if ((rc = x->upload((void **)&x->d_slot[d], ax.slot))) return rc;
+        if (d > 0) {  // y rows and z planes, packed
+            if (x->prec == PHPFDTD_FP64) rc = upload_rows<double>(x, ax, &x->d_axrow[d]);
+            else rc = upload_rows<float>(x, ax, &x->d_axrow[d]);
+            if (rc) return rc;
+        }
     }
     // slot lists of ADE / lattice materials, x-fastest order
     for (int m = 0; m < kMaxMat; ++m) x->special_of[m] = -1;
@@ -963,8 +936,12 @@ static void dft_install(phpfdtd_ctx *x, int comp, char axis, int p, int ref_comp
     DftArgs &a = x->dft;
     a = DftArgs{};
     const int emag = comp == PHPFDTD_EMAG;
-    a.F = x->F[emag ? 0 : comp];
-    a.Ex = x->F[0]; a.Ey = x->F[1]; a.Ez = x->F[2];
+    for (int set = 0; set < 2; ++set) {
+        a.F[set] = x->F[set][emag ? 0 : comp];
+        a.Ex[set] = x->F[set][0];
+        a.Ey[set] = x->F[set][1];
+        a.Ez[set] = x->F[set][2];
+    }
     a.emag = emag;
     a.ref_float = ref_compat ? 1 : 0;
     a.axis = axis;
@@ -1148,10 +1125,10 @@ static int sync_impl(phpfdtd_ctx *x, phpfdtd_diag *d) {
     unsigned long long init[4] = {0, 0, 0, ~0ull};
     HIPCHK(hipMemcpyAsync(x->d_diag, init, sizeof(init), hipMemcpyHostToDevice, x->stream));
     if (x->prec == PHPFDTD_FP64)
-        launch_reduce_e<double>((double *)x->F[0], (double *)x->F[1], (double *)x->F[2], x->nx, x->ny, x->nz, x->nxp,
+        launch_reduce_e<double>((double *)x->cur(0), (double *)x->cur(1), (double *)x->cur(2), x->nx, x->ny, x->nz, x->nxp,
                                 x->d_diag, x->stream);
     else
-        launch_reduce_e<float>((float *)x->F[0], (float *)x->F[1], (float *)x->F[2], x->nx, x->ny, x->nz, x->nxp,
+        launch_reduce_e<float>((float *)x->cur(0), (float *)x->cur(1), (float *)x->cur(2), x->nx, x->ny, x->nz, x->nxp,
                                x->d_diag, x->stream);
     // one pass over each lattice material's r: max |r| and its first argmax
     for (auto &sp : x->specials)
@@ -1203,10 +1180,10 @@ static int read_slice_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref
     const int emag = comp == PHPFDTD_EMAG;
     const int fc = emag ? 0 : comp;
     if (x->prec == PHPFDTD_FP64)
-        launch_slice<double>((double *)x->F[fc], (double *)x->F[0], (double *)x->F[1], (double *)x->F[2], emag, axis,
+        launch_slice<double>((double *)x->cur(fc), (double *)x->cur(0), (double *)x->cur(1), (double *)x->cur(2), emag, axis,
                              p, x->nx, x->ny, x->nz, x->nxp, (float *)buf, x->stream);
     else
-        launch_slice<float>((float *)x->F[fc], (float *)x->F[0], (float *)x->F[1], (float *)x->F[2], emag, axis, p,
+        launch_slice<float>((float *)x->cur(fc), (float *)x->cur(0), (float *)x->cur(1), (float *)x->cur(2), emag, axis, p,
                             x->nx, x->ny, x->nz, x->nxp, (float *)buf, x->stream);
     HIPCHK(hipMemcpyAsync(out, buf, n * sizeof(float), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1246,10 +1223,10 @@ static int read_point_impl(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
     const size_t c = x->fidx(px, py, pz);
     for (int q = 0; q < 9; ++q) {
         if (x->prec == PHPFDTD_FP64) {
-            HIPCHK(hipMemcpy(&f9[q], (double *)x->F[q] + c, 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&f9[q], (double *)x->cur(q) + c, 8, hipMemcpyDeviceToHost));
         } else {
             float v;
-            HIPCHK(hipMemcpy(&v, (float *)x->F[q] + c, 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&v, (float *)x->cur(q) + c, 4, hipMemcpyDeviceToHost));
             f9[q] = v;
         }
     }
@@ -1304,7 +1281,7 @@ static int read_field_impl(phpfdtd_ctx *x, int comp, double *out) {
     HIPCHK(hipSetDevice(x->device));
     HIPCHK(hipStreamSynchronize(x->stream));
     std::vector<unsigned char> h(x->ncellp * x->rs);
-    HIPCHK(hipMemcpy(h.data(), x->F[comp], h.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data(), x->cur(comp), h.size(), hipMemcpyDeviceToHost));
     for (int i = 0; i < x->nx; ++i)
         for (int j = 0; j < x->ny; ++j)
             for (int k = 0; k < x->nz; ++k) {
@@ -1332,7 +1309,11 @@ static int write_field_impl(phpfdtd_ctx *x, int comp, const double *in) {
                 if (x->prec == PHPFDTD_FP64) ((double *)h.data())[c] = v;
                 else ((float *)h.data())[c] = (float)v;
             }
-    HIPCHK(hipMemcpy(x->F[comp], h.data(), h.size(), hipMemcpyHostToDevice));
+    // E and H go to both sets: cells the update never changes (H at i = nx-1
+    // or j = ny-1, fdtd.c:3123-3124) then hold the same value in either
+    HIPCHK(hipMemcpy(x->F[0][comp], h.data(), h.size(), hipMemcpyHostToDevice));
+    if (x->F[1][comp] != x->F[0][comp])
+        HIPCHK(hipMemcpy(x->F[1][comp], h.data(), h.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1566,6 +1547,7 @@ int phpfdtd_struct_sizes(int *sizes, int n) {
 struct phpfdtd_batch {
     int prec = 0;  // field form of the members (PHPFDTD_FP64 / PHPFDTD_FP32)
     int form = 0;  // the members' precision value (PHPFDTD_FP64 / FP32 / FP32_RV64)
+    bool fused = true;  // Yee plan of the members (they share it)
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
@@ -1736,8 +1718,12 @@ static void batch_step_t(phpfdtd_batch *b) {
         if constexpr (sizeof(real) == 8) return b->yee64;
         else return b->yee32;
     }();
-    launch_h_batch<real>(y, b->stream);
-    launch_de_batch<real>(y, b->max_nmat, b->stream);
+    if (b->fused) {
+        launch_yee_batch<real>(y, b->max_nmat, b->stream);
+    } else {
+        launch_h_batch<real>(y, b->stream);
+        launch_de_batch<real>(y, b->max_nmat, b->stream);
+    }
     for (auto &g : b->groups) {
         if constexpr (sizeof(real) == 8) launch_slot_batch<real>(g.b64, g.kind, b->stream);
         else launch_slot_batch<real>(g.b32, g.kind, b->stream);
@@ -1777,11 +1763,9 @@ static int batch_add_impl(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (x->g.precision != b->form)
         return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->g.precision, b->form);
     if (x->device != b->device) return fail(PHPFDTD_EINVAL, "context is on device %d, batch on %d", x->device, b->device);
+    if (b->members.empty()) b->fused = x->fused;
+    if (x->fused != b->fused) return fail(PHPFDTD_EINVAL, "context's Yee plan differs from the batch members'");
     HIPCHK(hipSetDevice(x->device));
-    if (x->yee_rows != 1) {  // batched Yee launches take one y row per wavefront
-        x->yee_rows = 1;
-        x->dirty = true;
-    }
     if (!x->ev) HIPCHK(hipEventCreateWithFlags(&x->ev, hipEventDisableTiming));
     b->members.push_back(x);
     x->batch = b;

@@ -59,18 +59,32 @@ struct Axis {
     const int *slot;  // psi slab index 0..2L-1, or -1 outside the CPML
 };
 
+// E and H are double-buffered: step s (the device step counter at the start
+// of the step) reads set s & 1 and writes set (s + 1) & 1, so a work-group
+// that recomputes a neighbour's H from old values never meets one that
+// another work-group has already replaced; after s steps set s & 1 is
+// current.  The H psi slabs follow H.  D and the D psi are read and written
+// by their own cell only and stay single.
+// CPML coefficients of one y row or z plane, packed so that one scalar load
+// brings all of them (same meaning as the Axis tables).
+template <typename real>
+struct alignas(16) AxRow {
+    real kh, kd, b, ch, cd;
+    int p;  // psi slab index, -1 outside the CPML
+};
+
 template <typename real>
 struct YeeArgs {
-    real *Ex, *Ey, *Ez, *Dx, *Dy, *Dz, *Hx, *Hy, *Hz;
-    real *pHxy, *pHxz, *pDxy, *pDxz;  // x slabs
-    real *pHyx, *pHyz, *pDyx, *pDyz;  // y slabs
-    real *pHzx, *pHzy, *pDzx, *pDzy;  // z slabs
+    real *E[2][3], *H[2][3];        // [set][x, y, z]
+    real *Dx, *Dy, *Dz;
+    real *pH[2][6];                 // [set][xy, xz (x slabs), yx, yz (y slabs), zx, zy (z slabs)]
+    real *pDxy, *pDxz, *pDyx, *pDyz, *pDzx, *pDzy;
     const uint8_t *medium;
     const MatCoef<real> *mats;
-    Axis<real> ax, ay, az;
+    Axis<real> ax;                 // x: per-cell tables, read as lane vectors
+    const AxRow<real> *ay, *az;    // y rows, z planes
     int nx, ny, nz, nxp, L;
     int nmat;  // entries of mats
-    int rows;  // y rows per wavefront (1, 2 or 4)
     int kbase, kend;  // z planes [kbase, kend) of this launch
     int kchunk, nbx, nby, nbz, nblocks, nblocks_pad;
     // sources: cell and component (i, j, k, 0|1|2) in src[4s..4s+3], value*dt
@@ -182,8 +196,9 @@ __host__ __device__ inline void pk_pack(double x, float *hi, int *lo) {
 // Lattice / ADE slot kernels: one material per launch.
 template <typename real>
 struct SlotArgs {
-    real *Ex, *Ey, *Ez;
+    real *E[2][3];            // double-buffered E (YeeArgs): read set step & 1, write the other
     const real *Dx, *Dy, *Dz;
+    const int64_t *step;
     const uint32_t *cells;  // slot -> field index
     int64_t nslot;
     // crystal-frame coefficients
@@ -221,7 +236,7 @@ struct StepEndArgs {
     int64_t ring_rows;
     int mon_on;
     int64_t mon_c;     // field index sampled (z+1 in reference mode), -1 = zero halo
-    const void *F[9];  // field arrays (real)
+    const void *F[2][9];  // field arrays (real) of both E/H sets; the D entries are the same in both
     const void *mon_r, *mon_v;    // lattice state of the monitor cell's material
     int64_t mon_ridx, mon_vidx;   // elements of (slot, atom*3+dir) in r and v, -1 = no R/V
     int64_t mon_rc, mon_vc;       // their compensation bytes (compensated state), else -1
@@ -265,7 +280,7 @@ __device__ __forceinline__ A batch_args(const BatchLaunch<A> &b, int rid) {
 
 // Streamed single-bin DFT of one plane (phpfdtd_set_dft).
 struct DftArgs {
-    const void *F, *Ex, *Ey, *Ez;  // component sampled (real), and E for |E|
+    const void *F[2], *Ex[2], *Ey[2], *Ez[2];  // component sampled (real), and E for |E|, per E/H set
     int emag, ref_float;
     char axis;
     int pos;                       // plane index actually read (z+1 applied)

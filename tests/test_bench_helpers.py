@@ -12,7 +12,7 @@ sys.path.insert(0, ROOT)
 
 import bench  # noqa: E402
 
-PLAN = ["h_update", "de_update", "ade_update[m5]", "ion_update[m4,N16]", "step_end"]
+PLAN = ["yee_update", "ade_update[m5]", "ion_update[m4,N16]", "step_end"]
 
 
 def profile_symbols(section):

@@ -235,16 +235,15 @@ def ragged_case(shape, L=3):
     return spec, med
 
 
-@pytest.mark.parametrize("overlap", ["0", "1"])
+@pytest.mark.parametrize("plan", ["fused", "split"])
 @pytest.mark.parametrize("shape,L", [((300, 13, 21), 3), ((257, 9, 18), 3), ((264, 12, 20), 4)])
-def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, overlap):
+def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, plan):
     """Every field, lattice and ADE state of every form against the oracle
-    from a random state on grids with ragged x tiles, y blocks and z chunks;
-    with PHPFDTD_OVERLAP=1 the lattice slab also splits the H update into
-    three z ranges (chunks counted from each range's first plane)."""
-    monkeypatch.setenv("PHPFDTD_OVERLAP", overlap)
+    from a random state on grids with ragged x tiles, y blocks and z chunks,
+    under both Yee plans (one-pass, and H then D/E)."""
+    monkeypatch.setenv("PHPFDTD_YEE", plan)
     spec, med = ragged_case(shape, L)
-    print("ragged case", shape, "L", L, "overlap", overlap, flush=True)
+    print("ragged case", shape, "L", L, "plan", plan, flush=True)
     o = orc.Oracle(spec, med)
     for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL), (_abi.FP32_RV64, FP32_TOL)):
         sim = Simulation(spec, med, prec)
@@ -312,14 +311,12 @@ def test_compensated_vs_fp64_state(name):
     assert abs(a[2].max_r - b[2].max_r) <= 1e-5 * max(b[2].max_r, 1e-300)
 
 
-@pytest.mark.parametrize("overlap", ["0", "1"])
+@pytest.mark.parametrize("plan", ["fused", "split"])
 @pytest.mark.parametrize("case", ["hbn", "tip_slab/8"])
-def test_graph_matches_stream_launches(monkeypatch, case, overlap):
+def test_graph_matches_stream_launches(monkeypatch, case, plan):
     """16-step graphs give the same bits as plain sequential launches, with
-    and without an ADE material beside the lattice: the default plan, and the
-    overlapped one (PHPFDTD_OVERLAP=1: the next step's lattice-free H ranges
-    on a second stream beside the lattice kernel)."""
-    monkeypatch.setenv("PHPFDTD_OVERLAP", overlap)
+    and without an ADE material beside the lattice, under both Yee plans."""
+    monkeypatch.setenv("PHPFDTD_YEE", plan)
     if case == "hbn":
         c = golden_cases.load("hbn")
         spec, med, precs = c["spec"], c["medium"], (_abi.FP64,)
@@ -408,3 +405,48 @@ def test_fp32_ade_long_run():
     err = rel_l2(rows[_abi.FP32][:, 1:10], rows[_abi.FP64][:, 1:10])
     print("ADE 10000 steps FP32 vs FP64 rel L2", err, flush=True)
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("case", ["hbn", "aniso_loss", "ragged", "tip_slab/8"])
+def test_fused_plan_matches_split(monkeypatch, case):
+    """The one-pass Yee kernel recomputes the new H of the cells around each
+    wavefront (row j-1, plane k0-1, cell x0-1) with the code that computes
+    them in their owner, so it gives the same bits as writing H and reading it
+    back (PHPFDTD_YEE=split): every field, lattice and ADE state, from a
+    random state and then driven by the sources, in every form."""
+    if case == "ragged":
+        spec, med = ragged_case((300, 13, 21), 3)
+    elif case == "tip_slab/8":
+        spec, med = workloads.tip_slab(8)
+        spec = workloads.with_steps(spec, 40)
+    else:
+        c = golden_cases.load(case)
+        spec, med = c["spec"], c["medium"]
+    print("fused vs split", case, flush=True)
+    for prec in (_abi.FP64, _abi.FP32, _abi.FP32_RV64):
+        res = []
+        for plan in ("fused", "split"):
+            monkeypatch.setenv("PHPFDTD_YEE", plan)
+            sim = Simulation(spec, med, prec)
+            randomize(spec, med, 3, sim, orc.Oracle(spec, med) if case != "tip_slab/8" else _Sink())
+            sim.run(21)
+            snap = [sim.field(q) for q in range(9)]
+            for m, mat in enumerate(spec["materials"]):
+                if sim.count_cells(m) == 0:
+                    continue
+                for kind in ([0, 1] if mat["natom"] else []) + ([2, 3] if mat["use_ade"] else []):
+                    snap.append(sim.state(m, kind))
+            res.append(snap)
+            sim.close()
+        for a, b in zip(*res):
+            np.testing.assert_array_equal(a, b)
+
+
+class _Sink:
+    """Stands in for the oracle where only the GPU state is randomised."""
+
+    def set_field(self, *a):
+        pass
+
+    def set_state(self, *a):
+        pass
