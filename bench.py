@@ -284,6 +284,27 @@ def serving(n, step_scale, seed=4, load=0.8):
     }
 
 
+def serving_batched(n, step_scale, seed=4):
+    """The FP32 batched half of the serving block alone: every request of the
+    serving mix queued at t=0 and stepped through phpfdtd_batch_run (batched
+    launches only, no per-context graphs), with the step-model bytes of all
+    requests, so a kernel trace of this run prices the batched kernels."""
+    from php_fdtd_amd import _abi, serve, workloads
+
+    descs = workloads.serving_stream(n, seed=seed, step_scale=step_scale)
+    cases = [workloads.serving_case(d) for d in descs]
+    reqs = [serve.Request(d["id"], sp, med, arrival=0.0) for d, (sp, med) in zip(descs, cases)]
+    srv = serve.Server(_abi.FP32, progress=30)
+    st = serve.summarize(srv.serve(reqs))
+    members = [m for _, m, _ in srv.trace]
+    return dict(st, workload="serving mix, %d requests (seed %d), steps U(5000, 20000) x %g, FP32 form batched, "
+                             "all queued at t=0" % (n, seed, step_scale),
+                cell_updates=float(sum(med.size * sp["grid"]["tsteps"] for sp, med in cases)),
+                step_model_bytes=float(sum(sp["grid"]["tsteps"] * workloads.step_bytes(sp, med, 4, 5)
+                                           for sp, med in cases)),
+                mean_members=round(float(np.mean(members)), 2), max_members=int(max(members)))
+
+
 def host_info():
     """CPU model, physical cores, the CPU share of this process, host RAM and
     the MPI implementation, for the CPU baseline record."""
@@ -467,7 +488,10 @@ def main():
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false",
                     help="skip the measured copy-bandwidth ceilings")
     ap.add_argument("--serving-only", action="store_true",
-                    help="run only the serving block and print it as the JSON line (for profiling the batched path)")
+                    help="run only the serving block and print it as the JSON line")
+    ap.add_argument("--serving-batched-only", action="store_true",
+                    help="run only the FP32 batched serving run (all requests queued) and print it as the JSON "
+                         "line, with the step-model bytes of the requests (for profiling the batched kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; the domain is resident on a single MI355X and no "
@@ -485,6 +509,9 @@ def main():
 
     if args.serving_only:
         print(json.dumps({"serving": serving(args.serving, args.serving_step_scale)}))
+        return 0
+    if args.serving_batched_only:
+        print(json.dumps({"serving_batched": serving_batched(args.serving, args.serving_step_scale)}))
         return 0
     prof_steps = 3
     t0 = time.time()
