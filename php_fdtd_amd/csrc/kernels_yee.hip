@@ -15,9 +15,11 @@
 // Two plans give the same bits:
 //   * fused (yee_update, the default): one pass.  At each plane a wavefront
 //     computes the new H of its cells, then their new D and the E of plain
-//     cells.  D needs the new H of neighbours outside the wavefront - row
-//     j-1, plane k0-1 at the start of the chunk, and cell x0-1 for lane 0 -
-//     which it recomputes from the old E, H and psi with the same code
+//     cells.  The four wavefronts of a work-group hold four consecutive rows
+//     and pass E and the new H of their rows to the neighbouring wavefront
+//     through LDS.  D needs the new H of neighbours outside the work-group -
+//     row j0-1, plane k0-1 at the start of the chunk, and cell x0-1 for lane
+//     0 - which it recomputes from the old E, H and psi with the same code
 //     (h_cells) their owner runs; the old values cannot have been replaced by
 //     then, since new values go to the other set.  E, H and D each move once
 //     per cell and step: 18 words plus the material id.
@@ -558,16 +560,30 @@ __device__ __forceinline__ TileOf tile_of(const YeeArgs<real> &a, int t) {
 }
 
 // Fused plan: H, then D and E, plane by plane, for tile t of one domain.
+// The four wavefronts of a work-group hold four consecutive rows and trade
+// neighbour rows through LDS at one barrier per plane: a wavefront posts E of
+// plane k+1 (row j+1's H input at the next plane, for the wavefront below)
+// and its new H of plane k (row j-1's D input, for the wavefront above).
+// Wavefront 0 recomputes the new H of row j0-1 and wavefront 3 loads E of
+// row j0+4; every slot is double-buffered by plane parity.
+template <typename real>
+struct YeeRows {
+    typename Vec<real>::type ex[2][kBlockY][64], ez[2][kBlockY][64];  // E of plane k (parity k & 1)
+    typename Vec<real>::type hx[2][kBlockY][64], hz[2][kBlockY][64];  // new H of plane k
+};
 template <typename real>
 __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, const int t, const MatCoef<real> *mats) {
     typedef typename Vec<real>::type V;
     constexpr int VN = Vec<real>::n;
+    __shared__ YeeRows<real> rows;
     const TileOf tl = tile_of(a, t);
     const int j = tl.j;
+    const int w = threadIdx.y;
     const int lane = threadIdx.x;
     const int x0 = (tl.bx * 64 + lane) * VN;
-    if (j >= a.ny || uniform(tl.bx * 64 * VN) >= a.nx) return;
-    const bool store = x0 < a.nx;  // the lane holds cells (lanes past the row end still load and shuffle)
+    if (uniform(tl.bx * 64 * VN) >= a.nx) return;  // whole work-group
+    const bool live = j < a.ny;  // rows past the grid keep the barriers only
+    const bool store = live && x0 < a.nx;  // the lane holds cells (lanes past the row end still load and shuffle)
     const int k0 = uniform(a.kbase + tl.bz * a.kchunk);
     const int k1 = min(k0 + a.kchunk, a.kend);
     const size_t nxp = a.nxp;
@@ -575,87 +591,122 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, const int t, co
     const int64_t st = sld(a.step, 0);
     const Sets<real> s = sets(a, st);
     const LaneX<real> lx = lane_x(a, x0);
-    const bool hrow = j < a.ny - 1;              // H loop row
-    const bool drow = j >= 1 && j <= a.ny - 2;   // D loop row
-    const AxC<real> yc = axc(a.ay, j);
-    const AxC<real> yh = axc(a.ay, drow ? j - 1 : j);
+    const bool hrow = live && j < a.ny - 1;              // H loop row
+    const bool drow = live && j >= 1 && j <= a.ny - 2;   // D loop row
+    const bool first = w == 0, last = w == kBlockY - 1;
+    const int jj = live ? j : 0;
+    const AxC<real> yc = axc(a.ay, jj);
+    const AxC<real> yh = axc(a.ay, drow ? j - 1 : jj);
     const bool edge = lane == 63 && x0 + VN < a.nxp;  // loads the cell after its vector
     const bool xhalo = lane == 0 && x0 >= VN;         // recomputes the new H of cell x0-1
 
-    size_t c = ((size_t)k0 * a.ny + j) * nxp + lx.xr;
-    V ex = ld<V>(s.Eo[0] + c), ey = ld<V>(s.Eo[1] + c), ez = ld<V>(s.Eo[2] + c);
-    V eyh = V(0);            // Ey of row j-1 at plane k
+    size_t c = ((size_t)k0 * a.ny + jj) * nxp + lx.xr;
+    V ex = V(0), ey = V(0), ez = V(0);
+    V eyh = V(0);              // Ey of row j-1 at plane k (first wavefront)
     V hxk = V(0), hyk = V(0);  // new Hx, Hy at plane k-1 (zero below k = 0)
-    if (drow) {
-        eyh = ld<V>(s.Eo[1] + c - nxp);
-        if (k0 > 0) {  // new Hx, Hy of (j, k0-1)
-            const size_t m = c - plane;
-            const V exm = ld<V>(s.Eo[0] + m), eym = ld<V>(s.Eo[1] + m), ezm = ld<V>(s.Eo[2] + m);
-            const V ezmj = ld<V>(s.Eo[2] + m + nxp);
-            real n = shfl_down1(ezm[0]);
-            if (edge) n = s.Eo[2][m + VN];
-            V hz0 = V(0);
-            hxk = ld<V>(s.Ho[0] + m);
-            hyk = ld<V>(s.Ho[1] + m);
-            h_cells<real, HX | HY, false>(a, s, lx, yc, axc(a.az, k0 - 1), j, k0 - 1, false, exm, eym, ezm, ex, ey,
-                                          V(0), ezmj, V(0), next_x<V>(ezm, n), hxk, hyk, hz0);
+    if (live) {
+        ex = ld<V>(s.Eo[0] + c), ey = ld<V>(s.Eo[1] + c), ez = ld<V>(s.Eo[2] + c);
+        if (drow) {
+            if (first) eyh = ld<V>(s.Eo[1] + c - nxp);
+            if (k0 > 0) {  // new Hx, Hy of (j, k0-1)
+                const size_t m = c - plane;
+                const V exm = ld<V>(s.Eo[0] + m), eym = ld<V>(s.Eo[1] + m), ezm = ld<V>(s.Eo[2] + m);
+                const V ezmj = ld<V>(s.Eo[2] + m + nxp);
+                real n = shfl_down1(ezm[0]);
+                if (edge) n = s.Eo[2][m + VN];
+                V hz0 = V(0);
+                hxk = ld<V>(s.Ho[0] + m);
+                hyk = ld<V>(s.Ho[1] + m);
+                h_cells<real, HX | HY, false>(a, s, lx, yc, axc(a.az, k0 - 1), j, k0 - 1, false, exm, eym, ezm, ex, ey,
+                                              V(0), ezmj, V(0), next_x<V>(ezm, n), hxk, hyk, hz0);
+            }
         }
     }
+    rows.ex[k0 & 1][w][lane] = ex;
+    rows.ez[k0 & 1][w][lane] = ez;
+    __syncthreads();
     for (int k = k0; k < k1; ++k, c += plane) {
+        const int p = k & 1;
         const AxC<real> zc = axc(a.az, k);
         const bool top = k + 1 >= a.nz;  // E above the top plane is the zero halo
         V exk = V(0), eyk = V(0), ezk = V(0);
-        if (!top) {
-            exk = ld<V>(s.Eo[0] + c + plane);
-            eyk = ld<V>(s.Eo[1] + c + plane);
-            ezk = ld<V>(s.Eo[2] + c + plane);
-        }
-        V hx = ld<V>(s.Ho[0] + c), hy = ld<V>(s.Ho[1] + c), hz = ld<V>(s.Ho[2] + c);
-        V dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
-        real ezn = shfl_down1(ez[0]), eyn = shfl_down1(ey[0]);
-        if (edge) {
-            ezn = s.Eo[2][c + VN];
-            eyn = s.Eo[1][c + VN];
-        }
-        if (hrow) {
-            const V exj = ld<V>(s.Eo[0] + c + nxp), ezj = ld<V>(s.Eo[2] + c + nxp);
-            h_cells<real, HX | HY | HZ, true>(a, s, lx, yc, zc, j, k, store, ex, ey, ez, exk, eyk, exj, ezj,
-                                              next_x<V>(ey, eyn), next_x<V>(ez, ezn), hx, hy, hz);
-            if (store) {
-                stnt(hx, s.Hn[0] + c);
-                stnt(hy, s.Hn[1] + c);
-                stnt(hz, s.Hn[2] + c);
+        V hx = V(0), hy = V(0), hz = V(0), dx = V(0), dy = V(0), dz = V(0);
+        real ezn = 0, eyn = 0;
+        if (live) {
+            if (!top) {
+                exk = ld<V>(s.Eo[0] + c + plane);
+                eyk = ld<V>(s.Eo[1] + c + plane);
+                ezk = ld<V>(s.Eo[2] + c + plane);
+            }
+            hx = ld<V>(s.Ho[0] + c), hy = ld<V>(s.Ho[1] + c), hz = ld<V>(s.Ho[2] + c);
+            dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+            ezn = shfl_down1(ez[0]), eyn = shfl_down1(ey[0]);
+            if (edge) {
+                ezn = s.Eo[2][c + VN];
+                eyn = s.Eo[1][c + VN];
+            }
+            if (hrow) {
+                V exj, ezj;
+                if (last) {
+                    exj = ld<V>(s.Eo[0] + c + nxp);
+                    ezj = ld<V>(s.Eo[2] + c + nxp);
+                } else {
+                    exj = rows.ex[p][w + 1][lane];
+                    ezj = rows.ez[p][w + 1][lane];
+                }
+                h_cells<real, HX | HY | HZ, true>(a, s, lx, yc, zc, j, k, store, ex, ey, ez, exk, eyk, exj, ezj,
+                                                  next_x<V>(ey, eyn), next_x<V>(ez, ezn), hx, hy, hz);
+                if (store) {
+                    stnt(hx, s.Hn[0] + c);
+                    stnt(hy, s.Hn[1] + c);
+                    stnt(hz, s.Hn[2] + c);
+                }
             }
         }
-        int mid[VN], fl[VN];
-        bool anyloss, allplain;
-        cell_mats(a, c, mats, mid, fl, anyloss, allplain);
-        V eyhk = V(0);
-        if (drow) {
-            // new Hx, Hz of row j-1
-            const size_t m = c - nxp;
-            const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
-            if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
-            real n = shfl_down1(eyh[0]);
-            if (edge) n = s.Eo[1][m + VN];
-            V hxj = ld<V>(s.Ho[0] + m), hyj = V(0), hzj = ld<V>(s.Ho[2] + m);
-            h_cells<real, HX | HZ, false>(a, s, lx, yh, zc, j - 1, k, false, exh, eyh, ezh, V(0), eyhk, ex, ez,
-                                          next_x<V>(eyh, n), V(0), hxj, hyj, hzj);
-            // new Hy, Hz of the i-1 neighbours: the previous lane's last cell;
-            // lane 0 recomputes the cell before its vector
-            real hyp = shfl_up1(hy[VN - 1]), hzp = shfl_up1(hz[VN - 1]);
-            if (xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyp, hzp);
-            d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
-                    prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
-            hxk = hx;
-            hyk = hy;
+        rows.hx[p][w][lane] = hx;
+        rows.hz[p][w][lane] = hz;
+        rows.ex[p ^ 1][w][lane] = exk;
+        rows.ez[p ^ 1][w][lane] = ezk;
+        __syncthreads();
+        if (live) {
+            int mid[VN], fl[VN];
+            bool anyloss, allplain;
+            cell_mats(a, c, mats, mid, fl, anyloss, allplain);
+            V eyhk = V(0);
+            if (drow) {
+                // new Hx, Hz of row j-1: the wavefront above, or recomputed by the first
+                V hxj, hzj;
+                if (first) {
+                    const size_t m = c - nxp;
+                    const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
+                    if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
+                    real n = shfl_down1(eyh[0]);
+                    if (edge) n = s.Eo[1][m + VN];
+                    V hyj = V(0);
+                    hxj = ld<V>(s.Ho[0] + m);
+                    hzj = ld<V>(s.Ho[2] + m);
+                    h_cells<real, HX | HZ, false>(a, s, lx, yh, zc, j - 1, k, false, exh, eyh, ezh, V(0), eyhk, ex,
+                                                  ez, next_x<V>(eyh, n), V(0), hxj, hyj, hzj);
+                } else {
+                    hxj = rows.hx[p][w - 1][lane];
+                    hzj = rows.hz[p][w - 1][lane];
+                }
+                // new Hy, Hz of the i-1 neighbours: the previous lane's last cell;
+                // lane 0 recomputes the cell before its vector
+                real hyp = shfl_up1(hy[VN - 1]), hzp = shfl_up1(hz[VN - 1]);
+                if (xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyp, hzp);
+                d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
+                        prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
+                hxk = hx;
+                hyk = hy;
+            }
+            inject(a, x0, j, k, st, dx, dy, dz);
+            if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
+            ex = exk;
+            ey = eyk;
+            ez = ezk;
+            eyh = eyhk;
         }
-        inject(a, x0, j, k, st, dx, dy, dz);
-        if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
-        ex = exk;
-        ey = eyk;
-        ez = ezk;
-        eyh = eyhk;
     }
 }
 
