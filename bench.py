@@ -93,6 +93,8 @@ def kernel_symbol(name, form):
             return "phpfdtd::ion_update_f64<%d>" % n
         return "phpfdtd::ion_update_mfma<%d, %s>" % (n, "true" if form == "fp32" else "false")
     base = name.split("[")[0]
+    if base == "yee_update":  # one-pass kernel, instantiated per rows per work-group
+        return "phpfdtd::yee_update<%s, %d>" % (real, int(name.split("[r")[1].rstrip("]")))
     return "phpfdtd::%s<%s>" % (base, real)
 
 

@@ -24,7 +24,8 @@
 //     then, since new values go to the other set.  E, H and D each move once
 //     per cell and step: 18 words plus the material id.
 //   * split (h_update, then de_update): the new H is written, then read back
-//     by the D/E kernel: 21 words per cell.
+//     by the D/E kernel: 21 words per cell.  Batched launches (many small
+//     domains in one grid) use this plan: its tiles are smaller.
 // The arithmetic is written with explicit fused multiply-adds and the file is
 // compiled without FP contraction, so the owner's H and every recomputation
 // of it round identically.
@@ -554,7 +555,7 @@ __device__ __forceinline__ TileOf tile_of(const YeeArgs<real> &a, int t) {
     TileOf r;
     r.bx = t % a.nbx;
     const int rest = t / a.nbx;
-    r.j = uniform((rest % a.nby) * kBlockY + threadIdx.y);
+    r.j = uniform((rest % a.nby) * a.rows + threadIdx.y);
     r.bz = rest / a.nby;
     return r;
 }
@@ -566,16 +567,26 @@ __device__ __forceinline__ TileOf tile_of(const YeeArgs<real> &a, int t) {
 // and its new H of plane k (row j-1's D input, for the wavefront above).
 // Wavefront 0 recomputes the new H of row j0-1 and wavefront 3 loads E of
 // row j0+4; every slot is double-buffered by plane parity.
-template <typename real>
+template <typename real, int ROWS>
 struct YeeRows {
-    typename Vec<real>::type ex[2][kBlockY][64], ez[2][kBlockY][64];  // E of plane k (parity k & 1)
-    typename Vec<real>::type hx[2][kBlockY][64], hz[2][kBlockY][64];  // new H of plane k
+    typename Vec<real>::type ex[2][ROWS][64], ez[2][ROWS][64];  // E of plane k (parity k & 1)
+    typename Vec<real>::type hx[2][ROWS][64], hz[2][ROWS][64];  // new H of plane k
 };
+// The argument block in the constant address space (kernel arguments).
 template <typename real>
-__device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, const int t, const MatCoef<real> *mats) {
+using cYA = const __attribute__((address_space(4))) YeeArgs<real> *;
+// `a` is the argument block and `base` its address in the constant address
+// space.  Inside the plane loop the block is read again through `base`, made
+// opaque to the compiler once per plane, so the fifty-odd pointers and sizes
+// it holds are scalar loads next to their use rather than scalar registers
+// held across the loop (which spilled: 172 spilled scalars, 490 lane reads
+// in the loop; 50 and 132 this way, and 127 instead of 141 vector registers
+// in FP64, 4 wavefronts per SIMD instead of 3).
+template <typename real, int ROWS>
+__device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base, const int t, const MatCoef<real> *mats) {
     typedef typename Vec<real>::type V;
     constexpr int VN = Vec<real>::n;
-    __shared__ YeeRows<real> rows;
+    __shared__ YeeRows<real, ROWS> rows;
     const TileOf tl = tile_of(a, t);
     const int j = tl.j;
     const int w = threadIdx.y;
@@ -593,7 +604,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, const int t, co
     const LaneX<real> lx = lane_x(a, x0);
     const bool hrow = live && j < a.ny - 1;              // H loop row
     const bool drow = live && j >= 1 && j <= a.ny - 2;   // D loop row
-    const bool first = w == 0, last = w == kBlockY - 1;
+    const bool first = w == 0, last = w == ROWS - 1;
     const int jj = live ? j : 0;
     const AxC<real> yc = axc(a.ay, jj);
     const AxC<real> yh = axc(a.ay, drow ? j - 1 : jj);
@@ -626,6 +637,12 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, const int t, co
     rows.ez[k0 & 1][w][lane] = ez;
     __syncthreads();
     for (int k = k0; k < k1; ++k, c += plane) {
+        cYA<real> ap = base;
+        asm volatile("" : "+s"(ap));
+        const YeeArgs<real> &a = *(const YeeArgs<real> *)ap;
+        const Sets<real> s = sets(a, st);
+        const AxC<real> yc = axc(a.ay, jj);
+        const AxC<real> yh = axc(a.ay, drow ? j - 1 : jj);
         const int p = k & 1;
         const AxC<real> zc = axc(a.az, k);
         const bool top = k + 1 >= a.nz;  // E above the top plane is the zero halo
@@ -825,17 +842,17 @@ __device__ __forceinline__ MatCoef<real> *stage_mats(const YeeArgs<real> &a) {
     const int nw = a.nmat * (int)(sizeof(MatCoef<real>) / 4);
     const uint32_t *src = reinterpret_cast<const uint32_t *>(a.mats);
     uint32_t *dst = reinterpret_cast<uint32_t *>(lds_raw);
-    for (int w = threadIdx.y * 64 + threadIdx.x; w < nw; w += 256) dst[w] = src[w];
+    for (int w = threadIdx.y * 64 + threadIdx.x; w < nw; w += 64 * (int)blockDim.y) dst[w] = src[w];
     __syncthreads();
     return reinterpret_cast<MatCoef<real> *>(lds_raw);
 }
 
-template <typename real>
-__global__ __launch_bounds__(256) void yee_update(YeeArgs<real> a) {
+template <typename real, int ROWS>
+__global__ __launch_bounds__(64 * ROWS) void yee_update(YeeArgs<real> a) {
     const MatCoef<real> *mats = stage_mats(a);
     const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
     if (t < 0) return;
-    yee_tile<real>(a, t, mats);
+    yee_tile<real, ROWS>(a, (cYA<real>)__builtin_amdgcn_kernarg_segment_ptr(), t, mats);
 }
 
 template <typename real>
@@ -855,16 +872,6 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a) {
 
 // Batched launches: the tiles of every domain of a batch in one grid.  The
 // XCD renumbering runs over the concatenated tiles.
-
-template <typename real>
-__global__ __launch_bounds__(256) void yee_update_batch(BatchLaunch<YeeArgs<real>> b) {
-    const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
-    if (g < 0) return;  // whole work-group
-    const int rid = (int)sld(b.map, g);
-    const YeeArgs<real> a = batch_args(b, rid);
-    const MatCoef<real> *mats = stage_mats(a);
-    yee_tile<real>(a, g - sld(b.start, rid), mats);
-}
 
 template <typename real>
 __global__ __launch_bounds__(256) void h_update_batch(BatchLaunch<YeeArgs<real>> b) {
@@ -888,7 +895,12 @@ __global__ __launch_bounds__(256) void de_update_batch(BatchLaunch<YeeArgs<real>
 template <typename real>
 void launch_yee(const YeeArgs<real> &a, hipStream_t s) {
     const size_t lds = (size_t)a.nmat * sizeof(MatCoef<real>);
-    hipLaunchKernelGGL(yee_update<real>, dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a);
+    switch (a.rows) {
+        case 6: hipLaunchKernelGGL((yee_update<real, 6>), dim3(a.nblocks_pad), dim3(64, 6), lds, s, a); return;
+        case 8: hipLaunchKernelGGL((yee_update<real, 8>), dim3(a.nblocks_pad), dim3(64, 8), lds, s, a); return;
+        case 12: hipLaunchKernelGGL((yee_update<real, 12>), dim3(a.nblocks_pad), dim3(64, 12), lds, s, a); return;
+        default: hipLaunchKernelGGL((yee_update<real, 4>), dim3(a.nblocks_pad), dim3(64, 4), lds, s, a);
+    }
 }
 
 template <typename real>
@@ -900,12 +912,6 @@ template <typename real>
 void launch_de(const YeeArgs<real> &a, hipStream_t s) {
     const size_t lds = (size_t)a.nmat * sizeof(MatCoef<real>);
     hipLaunchKernelGGL(de_update<real>, dim3(a.nblocks_pad), dim3(64, kBlockY), lds, s, a);
-}
-
-template <typename real>
-void launch_yee_batch(const BatchLaunch<YeeArgs<real>> &b, int max_nmat, hipStream_t s) {
-    const size_t lds = (size_t)max_nmat * sizeof(MatCoef<real>);
-    if (b.total > 0) hipLaunchKernelGGL(yee_update_batch<real>, dim3(b.total_pad), dim3(64, kBlockY), lds, s, b);
 }
 
 template <typename real>
@@ -925,8 +931,6 @@ template void launch_h<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_h<float>(const YeeArgs<float> &, hipStream_t);
 template void launch_de<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_de<float>(const YeeArgs<float> &, hipStream_t);
-template void launch_yee_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);
-template void launch_yee_batch<float>(const BatchLaunch<YeeArgs<float>> &, int, hipStream_t);
 template void launch_h_batch<double>(const BatchLaunch<YeeArgs<double>> &, hipStream_t);
 template void launch_h_batch<float>(const BatchLaunch<YeeArgs<float>> &, hipStream_t);
 template void launch_de_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);

@@ -29,7 +29,6 @@ namespace phpfdtd {
 template <typename real> void launch_yee(const YeeArgs<real> &, hipStream_t);
 template <typename real> void launch_h(const YeeArgs<real> &, hipStream_t);
 template <typename real> void launch_de(const YeeArgs<real> &, hipStream_t);
-template <typename real> void launch_yee_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
 template <typename real> void launch_h_batch(const BatchLaunch<YeeArgs<real>> &, hipStream_t);
 template <typename real> void launch_de_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
 template <typename real> void launch_slot_batch(const BatchLaunch<SlotArgs<real>> &, int, hipStream_t);
@@ -142,6 +141,7 @@ struct phpfdtd_ctx {
     int64_t lossy_interior = 0;  // interior cells of materials with eps_im loss
     bool fused = true;     // one-pass Yee kernel (yee_update), else h_update + de_update
     int yee_kchunk = 16;   // z planes per Yee tile
+    int yee_rows = 12;     // y rows per work-group of the one-pass kernel (4, 6, 8 or 12)
     // sources
     int nsrc = 0;
     int *d_src = nullptr;  // [kMaxSrc][4] (i, j, k, component)
@@ -321,8 +321,9 @@ template <> SlotArgs<float> &slot_args_of<float>(Special &sp) { return sp.sa32; 
 // Yee plan switches, read at creation and whenever the plan is rebuilt (so a
 // context can be timed under several settings): PHPFDTD_YEE=split runs H and
 // D/E as two kernels (same results) instead of the one-pass kernel;
-// PHPFDTD_YEE_KCHUNK sets the z planes per tile (defaults measured on MI355X,
-// 1024x1024x640).  A batch member keeps the plan it joined with.
+// PHPFDTD_YEE_KCHUNK sets the z planes per tile and PHPFDTD_YEE_ROWS the y rows
+// per work-group of the one-pass kernel (defaults measured on MI355X,
+// 1024x1024x640).  A batch member runs the batch's split plan instead.
 static void plan_env(phpfdtd_ctx *x) {
     if (x->batch) return;
     const char *yp = getenv("PHPFDTD_YEE");
@@ -331,6 +332,15 @@ static void plan_env(phpfdtd_ctx *x) {
     if (const char *e = getenv("PHPFDTD_YEE_KCHUNK")) {
         const int v = atoi(e);
         if (v >= 1) x->yee_kchunk = v;
+    }
+    // one context, plan rebuilt per setting, 16-plane chunks: FP32 12 rows
+    // 11.49 ms against 11.77 (4 rows), 12.90 (8: one work-group per CU) and
+    // 15.31 (6: uneven over the 4 SIMDs); FP64 8 rows 21.70 ms against 22.44
+    // (4) and 23.25 (12: LDS-bound to one work-group per CU)
+    x->yee_rows = x->prec == PHPFDTD_FP32 ? 12 : 8;
+    if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
+        const int v = atoi(e);
+        if (v == 4 || v == 6 || v == 8 || v == 12) x->yee_rows = v;
     }
 }
 
@@ -364,12 +374,14 @@ static void build_plan_t(phpfdtd_ctx *x) {
     a.az = (const AxRow<real> *)x->d_axrow[2];
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz; a.nxp = x->nxp; a.L = x->L;
     a.nmat = x->nmat;
-    // a work-group is 4 wavefronts, each one y row x 64 lanes x one 16-byte
-    // vector of x cells, marching over a short chunk of z planes (streams of
-    // many short-lived wavefronts run closest to the HBM peak)
+    // a work-group is `rows` wavefronts (4 in the split plan), each one y row
+    // x 64 lanes x one 16-byte vector of x cells, marching over a short chunk
+    // of z planes (streams of many short-lived wavefronts run closest to the
+    // HBM peak)
     const int vec = YeeVec<real>::value;
     a.nbx = (x->nx + 64 * vec - 1) / (64 * vec);
-    a.nby = (x->ny + kBlockY - 1) / kBlockY;
+    a.rows = x->fused ? x->yee_rows : kBlockY;
+    a.nby = (x->ny + a.rows - 1) / a.rows;
     const int kc = std::min(x->yee_kchunk, x->nz);
     a.kchunk = kc;
     a.kbase = 0;
@@ -400,7 +412,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
         // E read once (3w), E of plain cells written (3w), H read and written
         // (6w), D read and written (6w), material id, psi of both halves
         Kernel ky;
-        ky.name = "yee_update";
+        ky.name = "yee_update[r" + std::to_string(a.rows) + "]";  // rows per work-group
         ky.stage = K_YEE;
         ky.bytes = cells * (9 * w + 1) + (cells - special_cells) * 3 * w + hcells * 6 * w + psi_h + psi_d;
         ky.flops = hcells * 18 + icells * 18 + cells * 15;
@@ -1547,7 +1559,6 @@ int phpfdtd_struct_sizes(int *sizes, int n) {
 struct phpfdtd_batch {
     int prec = 0;  // field form of the members (PHPFDTD_FP64 / PHPFDTD_FP32)
     int form = 0;  // the members' precision value (PHPFDTD_FP64 / FP32 / FP32_RV64)
-    bool fused = true;  // Yee plan of the members (they share it)
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
@@ -1718,12 +1729,8 @@ static void batch_step_t(phpfdtd_batch *b) {
         if constexpr (sizeof(real) == 8) return b->yee64;
         else return b->yee32;
     }();
-    if (b->fused) {
-        launch_yee_batch<real>(y, b->max_nmat, b->stream);
-    } else {
-        launch_h_batch<real>(y, b->stream);
-        launch_de_batch<real>(y, b->max_nmat, b->stream);
-    }
+    launch_h_batch<real>(y, b->stream);
+    launch_de_batch<real>(y, b->max_nmat, b->stream);
     for (auto &g : b->groups) {
         if constexpr (sizeof(real) == 8) launch_slot_batch<real>(g.b64, g.kind, b->stream);
         else launch_slot_batch<real>(g.b32, g.kind, b->stream);
@@ -1763,8 +1770,13 @@ static int batch_add_impl(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (x->g.precision != b->form)
         return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->g.precision, b->form);
     if (x->device != b->device) return fail(PHPFDTD_EINVAL, "context is on device %d, batch on %d", x->device, b->device);
-    if (b->members.empty()) b->fused = x->fused;
-    if (x->fused != b->fused) return fail(PHPFDTD_EINVAL, "context's Yee plan differs from the batch members'");
+    // Batches run the split Yee plan (same results): on the serving mix
+    // (64 requests, 64^3 .. 256x256x160) it gives 22% more solves/s than the
+    // one-pass kernel, whose four-row work-groups and 16-plane chunks leave
+    // small domains with few tiles.  The member's own plan returns when it
+    // leaves the batch (its next rebuild).
+    x->fused = false;
+    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 4 : 16;
     HIPCHK(hipSetDevice(x->device));
     if (!x->ev) HIPCHK(hipEventCreateWithFlags(&x->ev, hipEventDisableTiming));
     b->members.push_back(x);

@@ -27,7 +27,7 @@ constexpr int kMaxMat = 64;     // material ids are stored as uint8
 constexpr int kMaxSrc = 8;      // sources handled inside the D/E kernel
 constexpr int kMaxAtom = 32;    // lattice atoms per cell
 constexpr int kPackedKind = 64; // batch kind offset of compensated-state lattice kernels
-constexpr int kBlockY = 4;      // wavefronts per Yee work-group (64 x 4 threads)
+constexpr int kBlockY = 4;      // wavefronts per work-group of the split-plan Yee kernels (64 x 4 threads)
 constexpr int kDiagWaves = 8192;  // wavefront partials of the one-pass lattice diagnostics
 constexpr double kEpsInf = 20.0;  // EPS_INF of the reference ADE (fdtd.c:202)
 
@@ -87,6 +87,7 @@ struct YeeArgs {
     int nmat;  // entries of mats
     int kbase, kend;  // z planes [kbase, kend) of this launch
     int kchunk, nbx, nby, nbz, nblocks, nblocks_pad;
+    int rows;  // y rows (wavefronts) per work-group
     // sources: cell and component (i, j, k, 0|1|2) in src[4s..4s+3], value*dt
     // per step in src_val[step*nsrc + s]
     int nsrc;

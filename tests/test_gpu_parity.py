@@ -407,13 +407,17 @@ def test_fp32_ade_long_run():
     assert err <= 1e-4
 
 
-@pytest.mark.parametrize("case", ["hbn", "aniso_loss", "ragged", "tip_slab/8"])
-def test_fused_plan_matches_split(monkeypatch, case):
-    """The one-pass Yee kernel recomputes the new H of the cells around each
-    wavefront (row j-1, plane k0-1, cell x0-1) with the code that computes
+@pytest.mark.parametrize("case,rows", [("hbn", 4), ("aniso_loss", 4)] +
+                         [(c, r) for c in ("ragged", "tip_slab/8") for r in (4, 6, 8, 12)])
+def test_fused_plan_matches_split(monkeypatch, case, rows):
+    """The one-pass Yee kernel passes E and the new H between the rows of a
+    work-group through LDS and recomputes the new H of the cells around the
+    work-group (row j0-1, plane k0-1, cell x0-1) with the code that computes
     them in their owner, so it gives the same bits as writing H and reading it
     back (PHPFDTD_YEE=split): every field, lattice and ADE state, from a
-    random state and then driven by the sources, in every form."""
+    random state and then driven by the sources, in every form, for every
+    work-group height (PHPFDTD_YEE_ROWS), including work-groups whose last
+    rows lie past the grid."""
     if case == "ragged":
         spec, med = ragged_case((300, 13, 21), 3)
     elif case == "tip_slab/8":
@@ -422,7 +426,8 @@ def test_fused_plan_matches_split(monkeypatch, case):
     else:
         c = golden_cases.load(case)
         spec, med = c["spec"], c["medium"]
-    print("fused vs split", case, flush=True)
+    print("fused vs split", case, "rows", rows, flush=True)
+    monkeypatch.setenv("PHPFDTD_YEE_ROWS", str(rows))
     for prec in (_abi.FP64, _abi.FP32, _abi.FP32_RV64):
         res = []
         for plan in ("fused", "split"):
