@@ -571,8 +571,9 @@ def main():
                                        "dispatch, copy-probe calibrated" if traffic else None},
         "step_roofline": {"bytes_per_step": sb, "achieved": round(sb / (r["ms"] * 1e-3) / 1e9, 1),
                           "frac": round(sb / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "model": "C(18w+1) + 8w*CPML incidences + 9w*ADE cells + 12Nb*lattice cells, b = %d "
-                                   "bytes per stored r or v value" % rv_bytes[head]},
+                          "model": ("C(18w+1) - 6w*plain cells (E-form: their D is not stored)" if head != "fp64"
+                                    else "C(18w+1)") + " + 8w*CPML incidences + 9w*ADE cells + 12Nb*lattice cells, "
+                                   "b = %d bytes per stored r or v value" % rv_bytes[head]},
         "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r["prof"]},
         "ms_per_step_with_sampling": round(r["sampled_ms"], 4),
     }

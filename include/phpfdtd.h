@@ -63,6 +63,13 @@ extern "C" {
                            and stored as compensated FP32 (an FP32 word plus an
                            8-bit compensation term: 32 significant bits)         */
 #define PHPFDTD_FP32_RV64 2  /* precision form with ion r,v stored in FP64      */
+/* Both precision forms update cells of plain (non-ADE, non-lattice)
+ * materials in E-form: E^{n+1} = E^n + M.dD, the same update as the
+ * reference's D^{n+1} = D^n + dD, E = M.D^{n+1} (fdtd.c:3238-3462), without
+ * storing D.  The D those cells report (read_field, read_point, read_slice,
+ * monitor rows, the DFT) is M^-1 E; a D written to them is not used (their
+ * state is E).  ADE and lattice cells, and materials whose M has no
+ * inverse, keep D as the reference does. */
 
 /* Field components (phpfdtd_read_field / write_field / read_slice). */
 #define PHPFDTD_EX 0
@@ -261,7 +268,9 @@ int  phpfdtd_batch_wait(phpfdtd_batch *b, int keep);
 int  phpfdtd_batch_size(const phpfdtd_batch *b);
 void phpfdtd_batch_destroy(phpfdtd_batch *b);
 
-/* Whole component in file order, FP64 (tests, checkpoints). */
+/* Whole component in file order, FP64 (tests, checkpoints).  In the
+ * precision forms D of plain cells reads back as M^-1 E and is not taken on
+ * write (see PHPFDTD_FP32_RV64 above). */
 int phpfdtd_read_field(phpfdtd_ctx *ctx, int comp, double *out);
 int phpfdtd_write_field(phpfdtd_ctx *ctx, int comp, const double *in);
 

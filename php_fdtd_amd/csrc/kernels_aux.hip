@@ -37,8 +37,17 @@ __device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool res
             row[10] = rv[0];
             row[11] = rv[1];
         }
-        if (dh)
+        if (dh) {
             for (int q = 3; q < 9; ++q) row[1 + q] = val(q);
+            if (a.mon_dfe && a.mon_c >= 0) {  // E-form plain cell: D = M^-1 E, as derive_d computes it
+                const double e0 = val(0), e1 = val(1), e2 = val(2);
+                for (int r = 0; r < 3; ++r) {
+                    const double *m = a.mon_dm + 3 * r;
+                    const double d = fma(m[2], e2, fma(m[1], e1, m[0] * e0));
+                    row[4 + r] = f64 ? d : (double)(float)d;
+                }
+            }
+        }
     }
     if (rest) *a.step = st + 1;
 }
@@ -55,6 +64,28 @@ __global__ void step_end_batch(const StepEndArgs *args, int n) {
     if (threadIdx.x != 0 || (int)blockIdx.x >= n) return;
     const StepEndArgs a = args[blockIdx.x];
     step_row(a, true, true, sizeof(real) == 8);
+}
+
+// D = M^-1 . E for the E-form plain cells of a box; their D is not stored by
+// the time step (D of other cells is, and is left as it is).
+template <typename real>
+__global__ __launch_bounds__(256) void derive_d(DeriveArgs a) {
+    const int set = (int)((*a.step + a.ahead) & 1);
+    const real *Ex = static_cast<const real *>(a.E[set][0]), *Ey = static_cast<const real *>(a.E[set][1]),
+               *Ez = static_cast<const real *>(a.E[set][2]);
+    const int64_t bx = a.i1 - a.i0, by = a.j1 - a.j0, bz = a.k1 - a.k0;
+    const int64_t n = bx * by * bz;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = a.i0 + q % bx, j = a.j0 + (q / bx) % by, k = a.k0 + q / (bx * by);
+        const size_t c = ((size_t)k * a.ny + j) * a.nxp + i;
+        const int m = a.medium[c];
+        if (!a.derive[m]) continue;
+        const double e0 = Ex[c], e1 = Ey[c], e2 = Ez[c];
+        for (int r = 0; r < 3; ++r) {
+            const double *dm = a.dm + 9 * m + 3 * r;
+            static_cast<real *>(a.D[r])[c] = (real)fma(dm[2], e2, fma(dm[1], e1, dm[0] * e0));
+        }
+    }
 }
 
 template <typename T>
@@ -321,6 +352,14 @@ void launch_slice(const real *F, const real *Ex, const real *Ey, const real *Ez,
 }
 
 template <typename real>
+void launch_derive(const DeriveArgs &a, hipStream_t s) {
+    const int64_t n = (int64_t)(a.i1 - a.i0) * (a.j1 - a.j0) * (a.k1 - a.k0);
+    if (n <= 0) return;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(derive_d<real>, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, s, a);
+}
+
+template <typename real>
 void launch_dft(const DftArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(dft_accum<real>, dim3((unsigned)((a.nplane + 255) / 256)), dim3(256), 0, s, a);
 }
@@ -338,6 +377,7 @@ template void launch_reduce_e<double>(const double *, const double *, const doub
                                       unsigned long long *, hipStream_t);
 template void launch_reduce_e<float>(const float *, const float *, const float *, int, int, int, int,
                                      unsigned long long *, hipStream_t);
+template void launch_derive<float>(const DeriveArgs &, hipStream_t);
 template void launch_dft<double>(const DftArgs &, hipStream_t);
 template void launch_dft<float>(const DftArgs &, hipStream_t);
 template void launch_slice<double>(const double *, const double *, const double *, const double *, int, char, int,

@@ -528,13 +528,80 @@ __device__ __forceinline__ void store_de(const YeeArgs<real> &a, const Sets<real
     }
 }
 
-// Material ids of the lane's cells and their flags.
+// E-form store (FP32, plain cells carry E alone): E^{n+1} = E^n + M . dD,
+// the increment rounded on its own before it is added.  Cells with
+// MF_DSTATE advance D (dx, dy, dz, loaded when the lane has one) by dD
+// instead, and those that are not slot cells take E = M . D^{n+1}.
 template <typename real>
-__device__ __forceinline__ void cell_mats(const YeeArgs<real> &a, size_t c, const MatCoef<real> *mats, int *mid,
-                                          int *fl, bool &anyloss, bool &allplain) {
-    typedef typename Vec<real>::ids IDS;
+__device__ __forceinline__ void store_ef(const YeeArgs<real> &a, const Sets<real> &s, size_t c,
+                                         const MatCoef<real> *mats, const int *mid, const int *fl, bool anyd,
+                                         typename Vec<real>::type ex, typename Vec<real>::type ey,
+                                         typename Vec<real>::type ez, typename Vec<real>::type dx,
+                                         typename Vec<real>::type dy, typename Vec<real>::type dz,
+                                         typename Vec<real>::type gx, typename Vec<real>::type gy,
+                                         typename Vec<real>::type gz) {
+    typedef typename Vec<real>::type V;
     constexpr int VN = Vec<real>::n;
-    const IDS idw = *reinterpret_cast<const IDS *>(a.medium + c);
+    if (!anyd) {
+        V nex, ney, nez;
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            const real *M = mats[mid[e]].M;
+            nex[e] = ex[e] + fma(M[0], gx[e], fma(M[1], gy[e], M[2] * gz[e]));
+            ney[e] = ey[e] + fma(M[3], gx[e], fma(M[4], gy[e], M[5] * gz[e]));
+            nez[e] = ez[e] + fma(M[6], gx[e], fma(M[7], gy[e], M[8] * gz[e]));
+        }
+        stnt(nex, s.En[0] + c);
+        stnt(ney, s.En[1] + c);
+        stnt(nez, s.En[2] + c);
+        return;
+    }
+    dx += gx;
+    dy += gy;
+    dz += gz;
+    stnt(dx, a.Dx + c);
+    stnt(dy, a.Dy + c);
+    stnt(dz, a.Dz + c);
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+        if (fl[e] & MF_SPECIAL) continue;
+        const real *M = mats[mid[e]].M;
+        real nx, ny, nz;
+        if (fl[e] & MF_DSTATE) {
+            nx = fma(M[0], dx[e], fma(M[1], dy[e], M[2] * dz[e]));
+            ny = fma(M[3], dx[e], fma(M[4], dy[e], M[5] * dz[e]));
+            nz = fma(M[6], dx[e], fma(M[7], dy[e], M[8] * dz[e]));
+        } else {
+            nx = ex[e] + fma(M[0], gx[e], fma(M[1], gy[e], M[2] * gz[e]));
+            ny = ey[e] + fma(M[3], gx[e], fma(M[4], gy[e], M[5] * gz[e]));
+            nz = ez[e] + fma(M[6], gx[e], fma(M[7], gy[e], M[8] * gz[e]));
+        }
+        s.En[0][c + e] = nx;
+        s.En[1][c + e] = ny;
+        s.En[2][c + e] = nz;
+    }
+}
+
+// Whether any of the lane's cells keeps D (E-form): bit id of the
+// per-domain material mask, for each id byte of the word.
+template <typename real>
+__device__ __forceinline__ bool any_dstate(const YeeArgs<real> &a, typename Vec<real>::ids idw) {
+    constexpr int VN = Vec<real>::n;
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < VN; ++e) any |= ((a.dmask >> ((idw >> (8 * e)) & 0xff)) & 1) != 0;
+    return any;
+}
+
+// Material ids of the lane's cells (one word) and their flags.
+template <typename real>
+__device__ __forceinline__ typename Vec<real>::ids cell_ids(const YeeArgs<real> &a, size_t c) {
+    return *reinterpret_cast<const typename Vec<real>::ids *>(a.medium + c);
+}
+template <typename real>
+__device__ __forceinline__ void cell_flags(typename Vec<real>::ids idw, const MatCoef<real> *mats, int *mid, int *fl,
+                                           bool &anyloss, bool &allplain) {
+    constexpr int VN = Vec<real>::n;
     anyloss = false;
     allplain = true;
 #pragma unroll
@@ -544,6 +611,11 @@ __device__ __forceinline__ void cell_mats(const YeeArgs<real> &a, size_t c, cons
         anyloss |= (fl[e] & MF_LOSS) != 0;
         allplain &= !(fl[e] & MF_SPECIAL);
     }
+}
+template <typename real>
+__device__ __forceinline__ void cell_mats(const YeeArgs<real> &a, size_t c, const MatCoef<real> *mats, int *mid,
+                                          int *fl, bool &anyloss, bool &allplain) {
+    cell_flags<real>(cell_ids(a, c), mats, mid, fl, anyloss, allplain);
 }
 
 // Tile t of the grid: x segment bx (64 lane vectors), row j, z chunk bz.
@@ -649,6 +721,18 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
         V exk = V(0), eyk = V(0), ezk = V(0);
         V hx = V(0), hy = V(0), hz = V(0), dx = V(0), dy = V(0), dz = V(0);
         real ezn = 0, eyn = 0;
+        typename Vec<real>::ids idw = 0;
+        bool anyd = false;  // E-form: the lane holds a cell that keeps D
+        // FP32: everything of plane k that depends only on old values is
+        // loaded and computed before the barrier, so that no load waits after
+        // it: the new Hx, Hz of row j-1 (first wavefront) and the new Hy, Hz
+        // of cell x0-1 (lane 0).  FP64 does those two after the barrier,
+        // where they hold fewer registers (125 VGPRs, 4 wavefronts per SIMD,
+        // against 137 and 3; its stream is near the copy rate either way).
+        constexpr bool early = EForm<real>::value;
+        V hxj = V(0), hzj = V(0);  // new Hx, Hz of row j-1
+        V eyhk = V(0);             // Ey of row j-1 at plane k+1 (first wavefront)
+        real hyx = 0, hzx = 0;     // new Hy, Hz of cell x0-1 (lane 0)
         if (live) {
             if (!top) {
                 exk = ld<V>(s.Eo[0] + c + plane);
@@ -656,11 +740,32 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
                 ezk = ld<V>(s.Eo[2] + c + plane);
             }
             hx = ld<V>(s.Ho[0] + c), hy = ld<V>(s.Ho[1] + c), hz = ld<V>(s.Ho[2] + c);
-            dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+            idw = cell_ids(a, c);
+            if constexpr (EForm<real>::value) {
+                anyd = any_dstate(a, idw);
+                if (anyd) dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+            } else {
+                dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+            }
             ezn = shfl_down1(ez[0]), eyn = shfl_down1(ey[0]);
             if (edge) {
                 ezn = s.Eo[2][c + VN];
                 eyn = s.Eo[1][c + VN];
+            }
+            if (early && drow) {
+                if (xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyx, hzx);
+                if (first) {
+                    const size_t m = c - nxp;
+                    const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
+                    if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
+                    real n = shfl_down1(eyh[0]);
+                    if (edge) n = s.Eo[1][m + VN];
+                    V hyj = V(0);
+                    hxj = ld<V>(s.Ho[0] + m);
+                    hzj = ld<V>(s.Ho[2] + m);
+                    h_cells<real, HX | HZ, false>(a, s, lx, yh, zc, j - 1, k, false, exh, eyh, ezh, V(0), eyhk, ex,
+                                                  ez, next_x<V>(eyh, n), V(0), hxj, hyj, hzj);
+                }
             }
             if (hrow) {
                 V exj, ezj;
@@ -688,12 +793,14 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
         if (live) {
             int mid[VN], fl[VN];
             bool anyloss, allplain;
-            cell_mats(a, c, mats, mid, fl, anyloss, allplain);
-            V eyhk = V(0);
+            cell_flags<real>(idw, mats, mid, fl, anyloss, allplain);
+            V gx = V(0), gy = V(0), gz = V(0);  // E-form: dD
             if (drow) {
                 // new Hx, Hz of row j-1: the wavefront above, or recomputed by the first
-                V hxj, hzj;
-                if (first) {
+                if (!first) {
+                    hxj = rows.hx[p][w - 1][lane];
+                    hzj = rows.hz[p][w - 1][lane];
+                } else if (!early) {
                     const size_t m = c - nxp;
                     const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
                     if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
@@ -704,21 +811,34 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
                     hzj = ld<V>(s.Ho[2] + m);
                     h_cells<real, HX | HZ, false>(a, s, lx, yh, zc, j - 1, k, false, exh, eyh, ezh, V(0), eyhk, ex,
                                                   ez, next_x<V>(eyh, n), V(0), hxj, hyj, hzj);
-                } else {
-                    hxj = rows.hx[p][w - 1][lane];
-                    hzj = rows.hz[p][w - 1][lane];
                 }
                 // new Hy, Hz of the i-1 neighbours: the previous lane's last cell;
                 // lane 0 recomputes the cell before its vector
                 real hyp = shfl_up1(hy[VN - 1]), hzp = shfl_up1(hz[VN - 1]);
-                if (xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyp, hzp);
-                d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
-                        prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
+                if (xhalo) {
+                    if (early) {
+                        hyp = hyx;
+                        hzp = hzx;
+                    } else {
+                        h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyp, hzp);
+                    }
+                }
+                if constexpr (EForm<real>::value)  // the increment dD, from zero
+                    d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
+                            prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, gx, gy, gz);
+                else
+                    d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
+                            prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
                 hxk = hx;
                 hyk = hy;
             }
-            inject(a, x0, j, k, st, dx, dy, dz);
-            if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
+            if constexpr (EForm<real>::value) {
+                inject(a, x0, j, k, st, gx, gy, gz);
+                if (store) store_ef(a, s, c, mats, mid, fl, anyd, ex, ey, ez, dx, dy, dz, gx, gy, gz);
+            } else {
+                inject(a, x0, j, k, st, dx, dy, dz);
+                if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
+            }
             ex = exk;
             ey = eyk;
             ez = ezk;
@@ -807,10 +927,22 @@ __device__ __forceinline__ void de_tile(const YeeArgs<real> &a, const int t, con
     }
     for (int k = k0; k < k1; ++k, c += plane) {
         const AxC<real> zc = axc(a.az, k);
-        V dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+        constexpr bool EF = EForm<real>::value;
         int mid[VN], fl[VN];
         bool anyloss, allplain;
         cell_mats(a, c, mats, mid, fl, anyloss, allplain);
+        // D-form: D, advanced in place.  E-form: D only for lanes with a cell
+        // that keeps it, the increment dD in g, and E^n of every cell.
+        const bool anyd = EF && any_dstate(a, *reinterpret_cast<const typename Vec<real>::ids *>(a.medium + c));
+        V dx = V(0), dy = V(0), dz = V(0), gx = V(0), gy = V(0), gz = V(0);
+        if (!EF || anyd) dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
+        V ex = V(0), ey = V(0), ez = V(0);
+        if (EF || (drow && anyloss)) {
+            ex = ld<V>(s.Eo[0] + c);
+            ey = ld<V>(s.Eo[1] + c);
+            ez = ld<V>(s.Eo[2] + c);
+        }
+        V &ux = EF ? gx : dx, &uy = EF ? gy : dy, &uz = EF ? gz : dz;  // what the curl advances
         if (drow) {
             const V hx = ld<V>(Hn[0] + c), hy = ld<V>(Hn[1] + c), hz = ld<V>(Hn[2] + c);
             const V hxj = ld<V>(Hn[0] + c - nxp), hzj = ld<V>(Hn[2] + c - nxp);
@@ -819,19 +951,16 @@ __device__ __forceinline__ void de_tile(const YeeArgs<real> &a, const int t, con
                 hyp = Hn[1][c - 1];
                 hzp = Hn[2][c - 1];
             }
-            V ex = V(0), ey = V(0), ez = V(0);
-            if (anyloss) {
-                ex = ld<V>(s.Eo[0] + c);
-                ey = ld<V>(s.Eo[1] + c);
-                ez = ld<V>(s.Eo[2] + c);
-            }
             d_cells(a, lx, yc, zc, j, k, store, mats, mid, fl, anyloss, hx, hy, hz, hxj, hzj, hxk, hyk,
-                    prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, dx, dy, dz);
+                    prev_x<V>(hy, hyp), prev_x<V>(hz, hzp), ex, ey, ez, ux, uy, uz);
             hxk = hx;
             hyk = hy;
         }
-        inject(a, x0, j, k, st, dx, dy, dz);
-        if (store) store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
+        inject(a, x0, j, k, st, ux, uy, uz);
+        if (store) {
+            if constexpr (EF) store_ef(a, s, c, mats, mid, fl, anyd, ex, ey, ez, dx, dy, dz, gx, gy, gz);
+            else store_de(a, s, c, mats, mid, fl, allplain, dx, dy, dz);
+        }
     }
 }
 

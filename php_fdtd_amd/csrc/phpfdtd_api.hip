@@ -39,6 +39,7 @@ template <typename real> void launch_ade(const SlotArgs<real> &, hipStream_t);
 template <typename real> void launch_ion(const SlotArgs<real> &, hipStream_t);
 template <typename real> void launch_step_end(const StepEndArgs &, hipStream_t);
 template <typename real> void launch_dft(const DftArgs &, hipStream_t);
+template <typename real> void launch_derive(const DeriveArgs &, hipStream_t);
 template <typename T> void launch_fill(T *, T, int64_t, hipStream_t);
 template <typename real>
 void launch_reduce_e(const real *, const real *, const real *, int, int, int, int, unsigned long long *,
@@ -139,6 +140,16 @@ struct phpfdtd_ctx {
     std::vector<Special> specials;
     int special_of[kMaxMat];
     int64_t lossy_interior = 0;  // interior cells of materials with eps_im loss
+    // E-form (FP32): plain cells carry E alone; materials with dstate keep D
+    // (slot materials, and plain ones whose M has no inverse).  dm: M^-1 of
+    // each E-form material, for reading D back.
+    bool eform = false;
+    std::vector<uint8_t> dstate, derive;
+    std::vector<double> dm;
+    uint64_t dmask = 0;
+    int64_t dstate_cells = 0;
+    double *d_dm = nullptr;
+    uint8_t *d_derive = nullptr;
     bool fused = true;     // one-pass Yee kernel (yee_update), else h_update + de_update
     int yee_kchunk = 16;   // z planes per Yee tile
     int yee_rows = 12;     // y rows per work-group of the one-pass kernel (4, 6, 8 or 12)
@@ -344,6 +355,32 @@ static void plan_env(phpfdtd_ctx *x) {
     }
 }
 
+// derive_d over the box [b0, b1) x [b2, b3) x [b4, b5) of cells, reading E
+// set (step + ahead) & 1.
+static DeriveArgs derive_args(const phpfdtd_ctx *x, const int b[6], int ahead) {
+    DeriveArgs d{};
+    for (int set = 0; set < 2; ++set)
+        for (int q = 0; q < 3; ++q) d.E[set][q] = x->F[set][q];
+    for (int q = 0; q < 3; ++q) d.D[q] = x->F[0][3 + q];
+    d.medium = x->d_medium;
+    d.dm = x->d_dm;
+    d.derive = x->d_derive;
+    d.nxp = x->nxp;
+    d.ny = x->ny;
+    d.i0 = b[0]; d.i1 = b[1]; d.j0 = b[2]; d.j1 = b[3]; d.k0 = b[4]; d.k1 = b[5];
+    d.ahead = ahead;
+    d.step = x->d_step;
+    return d;
+}
+
+// Between steps: D of the E-form plain cells of a box, written to the D
+// arrays on the context's stream before a reader copies them (D-form: no-op).
+static void derive_box(phpfdtd_ctx *x, int i0, int i1, int j0, int j1, int k0, int k1) {
+    if (!x->eform) return;
+    const int b[6] = {i0, i1, j0, j1, k0, k1};
+    launch_derive<float>(derive_args(x, b, 0), x->stream);
+}
+
 template <typename real>
 static void build_plan_t(phpfdtd_ctx *x) {
     plan_env(x);
@@ -393,6 +430,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
     a.src = x->d_src;
     a.src_val = (const real *)x->d_src_val;
     a.step = x->d_step;
+    a.dmask = x->dmask;
     args_of<real>(x) = a;
 
     // algorithmic bytes (compulsory traffic of each kernel's own streams)
@@ -408,13 +446,16 @@ static void build_plan_t(phpfdtd_ctx *x) {
     for (const auto &sp : x->specials) special_cells += (double)sp.cells.size();
     const double lossy_int = (double)x->lossy_interior;
     const double psi_h = pml_inc(nx - 1, ny - 1, nz) * 4 * w, psi_d = pml_inc(nx - 2, ny - 2, nz) * 4 * w;
+    // D read and written: every cell (D-form), cells that keep D (E-form)
+    const double dcells = x->eform ? (double)x->dstate_cells : cells;
     if (x->fused) {
-        // E read once (3w), E of plain cells written (3w), H read and written
-        // (6w), D read and written (6w), material id, psi of both halves
+        // E read once (3w), E of non-slot cells written (3w), H read and
+        // written (6w), D read and written (6w), material id, psi of both halves
         Kernel ky;
         ky.name = "yee_update[r" + std::to_string(a.rows) + "]";  // rows per work-group
         ky.stage = K_YEE;
-        ky.bytes = cells * (9 * w + 1) + (cells - special_cells) * 3 * w + hcells * 6 * w + psi_h + psi_d;
+        ky.bytes = cells * (3 * w + 1) + dcells * 6 * w + (cells - special_cells) * 3 * w + hcells * 6 * w + psi_h +
+                   psi_d;
         ky.flops = hcells * 18 + icells * 18 + cells * 15;
         ky.fn = [a](hipStream_t s) { launch_yee<real>(a, s); };
         x->plan.push_back(ky);
@@ -429,7 +470,9 @@ static void build_plan_t(phpfdtd_ctx *x) {
         Kernel kd;
         kd.name = "de_update";
         kd.stage = K_YEE;
-        kd.bytes = cells * (6 * w + 1) + (cells - special_cells) * 3 * w + icells * 3 * w + lossy_int * 3 * w + psi_d;
+        // E-form: E^n of every cell is read (it is advanced); D-form: only lossy cells' E
+        kd.bytes = cells * 1 + dcells * 6 * w + (cells - special_cells) * 3 * w + icells * 3 * w +
+                   (x->eform ? cells : lossy_int) * 3 * w + psi_d;
         kd.flops = icells * 18 + cells * 15;
         kd.fn = [a](hipStream_t s) { launch_de<real>(a, s); };
         x->plan.push_back(kd);
@@ -484,6 +527,20 @@ static void build_plan_t(phpfdtd_ctx *x) {
         x->plan.push_back(k);
         slot_args_of<real>(sp) = sa;
     }
+    if (x->dft_on && x->eform && x->dft_comp >= PHPFDTD_DX && x->dft_comp <= PHPFDTD_DZ) {
+        const DftArgs &da = x->dft;
+        int b[6] = {0, x->nx, 0, x->ny, 0, x->nz};
+        const int ax = da.axis == 'x' ? 0 : da.axis == 'y' ? 1 : 2;
+        b[2 * ax] = da.pos;
+        b[2 * ax + 1] = std::min(da.pos + 1, ax == 0 ? x->nx : ax == 1 ? x->ny : x->nz);
+        const DeriveArgs dv = derive_args(x, b, 1);  // inside the step: the E set just written
+        Kernel kv;
+        kv.name = "derive_d";
+        kv.bytes = (double)da.nplane * (3 * w + 1);
+        kv.stage = K_PRE;
+        kv.fn = [dv](hipStream_t s) { launch_derive<float>(dv, s); };
+        x->plan.push_back(kv);
+    }
     if (x->dft_on) {
         DftArgs da = x->dft;
         Kernel kf;
@@ -511,6 +568,11 @@ static void build_plan_t(phpfdtd_ctx *x) {
     se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
     se.part = 0;
+    se.mon_dfe = 0;
+    if (x->eform && x->mon_c >= 0 && x->derive[x->medium[x->mon_c]]) {
+        se.mon_dfe = 1;
+        memcpy(se.mon_dm, &x->dm[9 * (size_t)x->medium[x->mon_c]], sizeof(se.mon_dm));
+    }
     x->se = se;
     Kernel ke;
     ke.name = "step_end";
@@ -556,6 +618,37 @@ static std::vector<real> cast_vec(const std::vector<double> &v) {
     return std::vector<real>(v.begin(), v.end());
 }
 
+// Update form of each material.  The FP32 form runs plain cells in E-form
+// (E^{n+1} = E^n + M . dD, D not stored); slot materials keep D, and so does
+// a plain material whose M is singular (|det M| below 1e-12 of the product
+// of its row norms), since its D could not be recovered from E.
+static void eform_tables(phpfdtd_ctx *x, const std::vector<int64_t> &cnt) {
+    x->eform = x->prec == PHPFDTD_FP32;
+    x->dstate.assign(x->nmat, 1);
+    x->derive.assign(x->nmat, 0);
+    x->dm.assign(9 * (size_t)x->nmat, 0.0);
+    x->dmask = 0;
+    x->dstate_cells = 0;
+    for (int m = 0; m < x->nmat; ++m) {
+        const double *M = x->pre[m].M;
+        bool keep = !x->eform || x->pre[m].special;
+        if (!keep) {
+            const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                               M[2] * (M[3] * M[7] - M[4] * M[6]);
+            double nrm = 1.0;
+            for (int r = 0; r < 3; ++r)
+                nrm *= sqrt(M[3 * r] * M[3 * r] + M[3 * r + 1] * M[3 * r + 1] + M[3 * r + 2] * M[3 * r + 2]);
+            keep = !(fabs(det) > 1e-12 * nrm) || inverse3(M, &x->dm[9 * (size_t)m]) != 0;
+        }
+        x->dstate[m] = keep ? 1 : 0;
+        x->derive[m] = keep ? 0 : 1;
+        if (keep) {
+            x->dmask |= 1ull << m;
+            x->dstate_cells += cnt[m];
+        }
+    }
+}
+
 template <typename real>
 static int upload_mats(phpfdtd_ctx *x) {
     std::vector<MatCoef<real>> mc(x->nmat);
@@ -565,7 +658,8 @@ static int upload_mats(phpfdtd_ctx *x) {
             mc[m].M[e] = (real)x->pre[m].M[e];
             mc[m].Lk[e] = (real)x->pre[m].Lk[e];
         }
-        mc[m].flags = (x->pre[m].loss ? MF_LOSS : 0) | (x->pre[m].special ? MF_SPECIAL : 0);
+        mc[m].flags = (x->pre[m].loss ? MF_LOSS : 0) | (x->pre[m].special ? MF_SPECIAL : 0) |
+                      (x->dstate[m] ? MF_DSTATE : 0);
     }
     return x->upload(&x->d_mats, mc);
 }
@@ -696,9 +790,14 @@ static int create_impl(const phpfdtd_grid *g, const phpfdtd_material *mats, int 
             for (int k = 0; k < x->nz; ++k) x->medium[x->fidx(i, j, k)] = (uint8_t)col[k];
         }
     if ((rc = x->upload((void **)&x->d_medium, x->medium))) return rc;
+    eform_tables(x, cnt);
     if (x->prec == PHPFDTD_FP64) rc = upload_mats<double>(x);
     else rc = upload_mats<float>(x);
     if (rc) return rc;
+    if (x->eform) {
+        if ((rc = x->upload((void **)&x->d_dm, x->dm))) return rc;
+        if ((rc = x->upload((void **)&x->d_derive, x->derive))) return rc;
+    }
     const int ns[3] = {x->nx, x->ny, x->nz};
     const double ds[3] = {g->dx, g->dy, g->dz};
     for (int d = 0; d < 3; ++d) {
@@ -1191,6 +1290,9 @@ static int read_slice_impl(phpfdtd_ctx *x, int comp, char axis, int pos, int ref
     float *buf = x->d_slice;
     const int emag = comp == PHPFDTD_EMAG;
     const int fc = emag ? 0 : comp;
+    if (fc >= 3 && fc <= 5)
+        derive_box(x, axis == 'x' ? p : 0, axis == 'x' ? p + 1 : x->nx, axis == 'y' ? p : 0, axis == 'y' ? p + 1 : x->ny,
+                   axis == 'z' ? p : 0, axis == 'z' ? p + 1 : x->nz);
     if (x->prec == PHPFDTD_FP64)
         launch_slice<double>((double *)x->cur(fc), (double *)x->cur(0), (double *)x->cur(1), (double *)x->cur(2), emag, axis,
                              p, x->nx, x->ny, x->nz, x->nxp, (float *)buf, x->stream);
@@ -1229,6 +1331,7 @@ static int read_point_impl(phpfdtd_ctx *x, int px, int py, int pz, double *f9) {
     if (!x || !f9 || px < 0 || px >= x->nx || py < 0 || py >= x->ny || pz < 0 || pz > x->nz)
         return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
+    if (pz < x->nz) derive_box(x, px, px + 1, py, py + 1, pz, pz + 1);
     HIPCHK(hipStreamSynchronize(x->stream));
     for (int q = 0; q < 9; ++q) f9[q] = 0.0;
     if (pz == x->nz) return 0;
@@ -1291,6 +1394,7 @@ int phpfdtd_read_ions(phpfdtd_ctx *x, int px, int py, int pz, double *r, double 
 static int read_field_impl(phpfdtd_ctx *x, int comp, double *out) {
     if (!x || !out || comp < 0 || comp > 8) return fail(PHPFDTD_EINVAL, "bad arguments");
     HIPCHK(hipSetDevice(x->device));
+    if (comp >= 3 && comp <= 5) derive_box(x, 0, x->nx, 0, x->ny, 0, x->nz);
     HIPCHK(hipStreamSynchronize(x->stream));
     std::vector<unsigned char> h(x->ncellp * x->rs);
     HIPCHK(hipMemcpy(h.data(), x->cur(comp), h.size(), hipMemcpyDeviceToHost));

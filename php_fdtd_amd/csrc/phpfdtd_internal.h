@@ -37,6 +37,21 @@ template <typename real> struct YeeVec { static constexpr int value = 16 / sizeo
 enum MatFlags : int {
     MF_LOSS = 1,     // eps_im != 0 and no ADE: D -= Lk . E^n (fdtd.c:3349-3353)
     MF_SPECIAL = 2,  // E is produced by a slot kernel (ADE and/or lattice)
+    MF_DSTATE = 4,   // FP32 form: the cell keeps D in HBM (slot cells, and plain
+                     // materials whose M has no inverse); other cells carry E alone
+};
+
+// Update form of the Yee kernels.  FP64 (the plain form) keeps the
+// reference's D-form for every cell: D += curl H (+ psi, loss, sources), E =
+// M . D (fdtd.c:3238-3462).  The FP32 precision form runs the same update
+// in E-form for plain cells: with dD the increment above, E^{n+1} = E^n +
+// M . dD, which equals M . (D^n + dD) since M is constant in time; their D
+// is then not stored (it is M^-1 E, derived when read).  That removes D's
+// read and write (6 words of 18) from every plain cell.  Cells with
+// MF_DSTATE keep D += dD, and the slot kernels read it.
+template <typename real>
+struct EForm {
+    static constexpr bool value = sizeof(real) == 4;
 };
 
 // Per-material coefficients read by the fused D/E kernel.
@@ -94,6 +109,7 @@ struct YeeArgs {
     const int *src;
     const real *src_val;
     const int64_t *step;
+    uint64_t dmask;  // bit m: material m has MF_DSTATE
 };
 
 // MFMA tiling of the lattice contraction -DMM . r for N atoms: k-tiles of 4
@@ -243,6 +259,24 @@ struct StepEndArgs {
     int64_t mon_rc, mon_vc;       // their compensation bytes (compensated state), else -1
     double dt;
     int part;                     // 0 whole row, 1 D and H, 2 t, E, R, V + step advance
+    int mon_dfe;                  // E-form plain monitor cell: D = mon_dm . E (D is not stored)
+    double mon_dm[9];
+};
+
+// D = Dm . E for the E-form plain cells of a box of cells (their D is not
+// stored), written to the D arrays so that readers find it there.  The E
+// set read is (step + ahead) & 1: ahead 1 inside a step's plan before the
+// counter advances, 0 between steps.
+struct DeriveArgs {
+    const void *E[2][3];
+    void *D[3];
+    const uint8_t *medium;
+    const double *dm;             // [nmat][9] D-from-E matrix (M^-1), FP64
+    const uint8_t *derive;        // [nmat] 1 = E-form plain material
+    int nxp, ny;
+    int i0, i1, j0, j1, k0, k1;   // box [i0, i1) x [j0, j1) x [k0, k1)
+    int ahead;
+    const int64_t *step;
 };
 
 // One launch over the work of several domains (a batch): tile g of the grid

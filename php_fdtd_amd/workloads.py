@@ -332,17 +332,23 @@ def cell_counts(spec, med):
     return counts, pml
 
 
-def step_bytes(spec, med, word, rv_bytes=8):
+def step_bytes(spec, med, word, rv_bytes=8, eform=None):
     """Algorithmic HBM bytes of one time step: each state word read and written
     once.  C (18w + 1) + 8w * CPML incidences + 9w * ADE cells
     + 12 N b * lattice cells, b the bytes per stored r or v value (8 for FP64,
-    5 for compensated FP32): 96 N or 60 N."""
+    5 for compensated FP32): 96 N or 60 N.  In E-form (the FP32 forms; the
+    default for w = 4) plain cells store E and H alone: 12w + 1 instead of
+    18w + 1 (their D is not stored)."""
+    if eform is None:
+        eform = word == 4
     counts, pml = cell_counts(spec, med)
     C = med.size
     b = C * (18 * word + 1) + 8 * word * pml
     for m, mat in enumerate(spec["materials"]):
         if counts[m] == 0:
             continue
+        if eform and not mat.get("use_ade") and not mat.get("natom", 0):
+            b -= 6 * word * counts[m]  # no D read or write
         if mat.get("use_ade"):
             b += 9 * word * counts[m]
         if mat.get("natom", 0):

@@ -6,6 +6,9 @@ Bars (stated here, checked below):
   * FP64 plain form vs the CPU oracle from random states (every term active):
     every field, lattice and ADE state within 1e-12 relative L2 per component.
   * FP32 precision form vs the golden FP64 traces: relative L2 <= 1e-4.
+  * Random states of the FP32 form are reachable ones (tests/states.py): the
+    form carries E alone for plain cells, so a state whose E is not M . D
+    there is not one it can hold.
 """
 import os
 
@@ -13,6 +16,7 @@ import numpy as np
 import pytest
 
 import golden_cases
+import states as reach_states
 from oracle import orc
 from php_fdtd_amd import _abi, workloads
 from php_fdtd_amd.sim import Simulation
@@ -104,31 +108,39 @@ def test_golden_fp32(golden):
         assert rel_l2(rows[:, 10:12], ref[:, 10:12]) <= FP32_TOL
 
 
-def randomize(spec, medium, seed, sim, o):
+def randomize(spec, medium, seed, sim, o, reach=False):
+    """Write one random state into sim and o: fields at the scales a run
+    reaches (D ~ eps0 E, H ~ E / 377) and every lattice and ADE state; with
+    reach, that state advanced one oracle step (E = M . D in plain cells)."""
     rng = np.random.default_rng(seed)
     shape = medium.shape
+    fields = []
     for comp in range(9):
         f = rng.standard_normal(shape)
         if comp >= 3 and comp < 6:
             f *= 8.854e-12  # D ~ eps0 E
         elif comp >= 6:
             f /= 377.0
-        sim.set_field(comp, f)
-        o.set_field(comp, f)
+        fields.append(f)
+    states = {}
     for m, mat in enumerate(spec["materials"]):
         n = sim.count_cells(m)
         if n == 0:
             continue
         if mat["natom"] > 0:
             for kind, scale in ((0, 1e-13), (1, 1e-3)):
-                a = rng.standard_normal(n * 3 * mat["natom"]) * scale
-                sim.set_state(m, kind, a)
-                o.set_state(m, kind, a)
+                states[(m, kind)] = rng.standard_normal(n * 3 * mat["natom"]) * scale
         if mat["use_ade"]:
             for kind in (2, 3):
-                a = rng.standard_normal(n * 3) * 1e-11
-                sim.set_state(m, kind, a)
-                o.set_state(m, kind, a)
+                states[(m, kind)] = rng.standard_normal(n * 3) * 1e-11
+    if reach:
+        fields, states = reach_states.reachable(spec, medium, fields, states)
+    for comp, f in enumerate(fields):
+        sim.set_field(comp, f)
+        o.set_field(comp, f)
+    for (m, kind), a in states.items():
+        sim.set_state(m, kind, a)
+        o.set_state(m, kind, a)
 
 
 @pytest.mark.parametrize("name", golden_cases.CASES)
@@ -189,9 +201,9 @@ def test_lattice_atom_counts(natom):
     forms = (_abi.FP64, _abi.FP32, _abi.FP32_RV64)
     sims = {p: Simulation(spec, med, p) for p in forms}
     o = orc.Oracle(spec, med)
-    randomize(spec, med, 99, sims[_abi.FP64], o)
+    randomize(spec, med, 99, sims[_abi.FP64], o, reach=True)
     for p in forms[1:]:
-        randomize(spec, med, 99, sims[p], orc.Oracle(spec, med))
+        randomize(spec, med, 99, sims[p], _Sink(), reach=True)
     for s in sims.values():
         s.run(5)
     o.step(5)
@@ -239,16 +251,15 @@ def ragged_case(shape, L=3):
 @pytest.mark.parametrize("shape,L", [((300, 13, 21), 3), ((257, 9, 18), 3), ((264, 12, 20), 4)])
 def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, plan):
     """Every field, lattice and ADE state of every form against the oracle
-    from a random state on grids with ragged x tiles, y blocks and z chunks,
-    under both Yee plans (one-pass, and H then D/E)."""
+    from a random (reachable) state on grids with ragged x tiles, y blocks and
+    z chunks, under both Yee plans (one-pass, and H then D/E)."""
     monkeypatch.setenv("PHPFDTD_YEE", plan)
     spec, med = ragged_case(shape, L)
     print("ragged case", shape, "L", L, "plan", plan, flush=True)
     o = orc.Oracle(spec, med)
     for prec, tol in ((_abi.FP64, RANDOM_TOL), (_abi.FP32, FP32_TOL), (_abi.FP32_RV64, FP32_TOL)):
         sim = Simulation(spec, med, prec)
-        oo = orc.Oracle(spec, med) if prec != _abi.FP64 else o
-        randomize(spec, med, 7, sim, oo)
+        randomize(spec, med, 7, sim, o if prec == _abi.FP64 else _Sink(), reach=True)
         if prec == _abi.FP64:
             o.step(6)
         sim.run(6)
