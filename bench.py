@@ -212,8 +212,10 @@ def accuracy(name, interval=10, steps=None, rv64=True):
         "nearfield": {"component": "Ez", "plane": "z=%d" % zplane, "samples": N, "interval": interval,
                       "bin": b["k"], "bin_hz": b["k"] * fs / N, "source_hz": f,
                       "amp_rel_l2": rl2(amp32, amp64),
-                      "phase_rms_rad": float(np.sqrt(np.sum(amp64 ** 2 * ph ** 2) / np.sum(amp64 ** 2))),
-                      "phase_max_rad_strong": float(np.abs(ph[strong]).max())},
+                      # (None where the plane holds no field at the bin)
+                      "phase_rms_rad": float(np.sqrt(np.sum(amp64 ** 2 * ph ** 2) / np.sum(amp64 ** 2)))
+                      if amp64.max() > 0 else None,
+                      "phase_max_rad_strong": float(np.abs(ph[strong]).max()) if amp64.max() > 0 else None},
         "solve_s": {"fp32": round(a["wall"], 3), "fp64": round(b["wall"], 3)},
     }
     if c:

@@ -27,6 +27,13 @@ printed output, never from a second copy of the oracle's formulas:
   * The ion equation of motion (fdtd.c:3505-3528) by a single-cell run
     against the per-mode 2x2 recurrences of the eigen-decomposed dynamical
     matrix, taken non-symmetric so that a transposed DMM . r fails.
+
+And one property of the reference's scheme that the workloads depend on:
+its CPML uses the old psi in the field update and advances it afterwards
+(fdtd.c:3152-3155), which is stable with its hard-coded kappa_max = 7
+(fdtd.c:976) and unstable with kappa_max = 1 (the rest of the profile of
+fdtd.c:962-989 unchanged): a vacuum run grows without bound from the PML
+corners.
 """
 import math
 import os
@@ -37,7 +44,7 @@ import pytest
 
 import golden_cases
 from oracle import orc
-from php_fdtd_amd import _abi
+from php_fdtd_amd import _abi, workloads
 
 EPS0 = 8.854e-12
 EPS_INF = 20.0  # fdtd.c:202
@@ -285,3 +292,21 @@ def test_ion_eom_modes_single_cell():
         assert np.linalg.norm(o.state(0, 0) - r) <= tol * np.linalg.norm(r), n
         assert np.linalg.norm(o.state(0, 1) - v) <= tol * np.linalg.norm(v), n
     assert np.all(o.field(0) == 0)  # no coupling to E
+
+
+@pytest.mark.parametrize("kappa_max,bounded", [(7.0, True), (1.0, False)])
+def test_lagged_cpml_needs_kappa(kappa_max, bounded):
+    """A Ricker dipole in a 32^3 vacuum box with a 10-cell CPML, 300 steps: at
+    kappa_max = 7 max|E| stays below 1e-4 (the pulse amplitude); at 1 it
+    passes 1e10 (the recursion's growing mode, about 1e40 per 100 steps)."""
+    n, L = 32, 10
+    f = 650.0 * workloads.WN_TO_HZ
+    g = workloads._grid((n, n, n), 20e-9, 300, L, f)
+    g["kappa_max"] = kappa_max
+    src = [workloads._source("ricker", (n // 2, n // 2, n // 2), "z", f, t0=5.128e-14, tsteps=300, dt=g["dt"])]
+    spec = dict(name="cpml_kappa", grid=g, materials=workloads.builtin_table(f), sources=src)
+    o = orc.Oracle(spec, np.zeros((n, n, n), np.int32))
+    o.step(300)
+    e = o.diag()[0][0]
+    o.close()
+    assert (e < 1e-4) if bounded else (e > 1e10 or not np.isfinite(e)), e
