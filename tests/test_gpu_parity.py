@@ -323,27 +323,44 @@ def test_compensated_vs_fp64_state(name):
 
 
 @pytest.mark.parametrize("plan", ["fused", "split"])
-@pytest.mark.parametrize("case", ["hbn", "tip_slab/8"])
+@pytest.mark.parametrize("case", ["hbn", "tip_slab/8", "tip_slab/8-air"])
 def test_graph_matches_stream_launches(monkeypatch, case, plan):
     """16-step graphs give the same bits as plain sequential launches, with
-    and without an ADE material beside the lattice, under both Yee plans."""
+    and without an ADE material beside the lattice, under both Yee plans and
+    in both forms: every field, lattice and ADE state and every monitor row
+    (the monitor in the lattice slab, and away from it)."""
     monkeypatch.setenv("PHPFDTD_YEE", plan)
     if case == "hbn":
         c = golden_cases.load("hbn")
-        spec, med, precs = c["spec"], c["medium"], (_abi.FP64,)
+        spec, med = c["spec"], c["medium"]
     else:  # 1024x1024x640 layout at 1/8 scale: lattice slab + Drude ADE cone
         spec, med = workloads.tip_slab(8)
-        spec, precs = workloads.with_steps(spec, 40), (_abi.FP64, _abi.FP32)
-    for prec in precs:
+        spec = workloads.with_steps(spec, 40)
+        if case.endswith("-air"):  # monitor in the planes the early launch updates
+            g = spec["grid"]
+            spec = dict(spec, monitor=dict(x=g["nx"] // 3, y=g["ny"] // 3, z=3, atom=0, dir=0))
+    mon = spec.get("monitor")
+    for prec in (_abi.FP64, _abi.FP32):
         res = []
         for ng in ("0", "1"):
             monkeypatch.setenv("PHPFDTD_NO_GRAPH", ng)
             sim = Simulation(spec, med, prec)
+            if mon:
+                sim.set_monitor(mon["x"], mon["y"], mon["z"], mon.get("atom", 0), mon.get("dir", 0))
             sim.run(37)  # 2 x 16-step graph + 5 x 1-step graph
-            res.append([sim.field(q) for q in range(9)])
+            snap = [sim.field(q) for q in range(9)]
+            for m, mat in enumerate(spec["materials"]):
+                if sim.count_cells(m) == 0:
+                    continue
+                for kind in ([0, 1] if mat["natom"] else []) + ([2, 3] if mat["use_ade"] else []):
+                    snap.append(sim.state(m, kind))
+            if mon:
+                snap.append(sim.monitor())
+            res.append(snap)
             sim.close()
-        for q in range(9):
-            np.testing.assert_array_equal(res[0][q], res[1][q])
+        assert len(res[0]) == len(res[1])
+        for a, b in zip(*res):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_errors_and_bounds():
