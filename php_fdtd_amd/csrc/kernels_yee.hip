@@ -723,12 +723,12 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
         real ezn = 0, eyn = 0;
         typename Vec<real>::ids idw = 0;
         bool anyd = false;  // E-form: the lane holds a cell that keeps D
-        // FP32: everything of plane k that depends only on old values is
-        // loaded and computed before the barrier, so that no load waits after
-        // it: the new Hx, Hz of row j-1 (first wavefront) and the new Hy, Hz
-        // of cell x0-1 (lane 0).  FP64 does those two after the barrier,
-        // where they hold fewer registers (125 VGPRs, 4 wavefronts per SIMD,
-        // against 137 and 3; its stream is near the copy rate either way).
+        // FP32: lane 0's new Hy, Hz of cell x0-1 depend only on old values
+        // and are loaded and computed before the barrier, so those loads do
+        // not wait after it.  The first wavefront's recomputed row j0-1 and
+        // the D of cells that keep it come after the barrier in both forms:
+        // held across it they cost 21 VGPRs, and the FP32 kernel fits 4
+        // wavefronts per SIMD (127 VGPRs) only without them.
         constexpr bool early = EForm<real>::value;
         V hxj = V(0), hzj = V(0);  // new Hx, Hz of row j-1
         V eyhk = V(0);             // Ey of row j-1 at plane k+1 (first wavefront)
@@ -741,32 +741,14 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
             }
             hx = ld<V>(s.Ho[0] + c), hy = ld<V>(s.Ho[1] + c), hz = ld<V>(s.Ho[2] + c);
             idw = cell_ids(a, c);
-            if constexpr (EForm<real>::value) {
-                anyd = any_dstate(a, idw);
-                if (anyd) dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
-            } else {
-                dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
-            }
+            if constexpr (EForm<real>::value) anyd = any_dstate(a, idw);  // D is loaded at the store
+            else dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
             ezn = shfl_down1(ez[0]), eyn = shfl_down1(ey[0]);
             if (edge) {
                 ezn = s.Eo[2][c + VN];
                 eyn = s.Eo[1][c + VN];
             }
-            if (early && drow) {
-                if (xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyx, hzx);
-                if (first) {
-                    const size_t m = c - nxp;
-                    const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
-                    if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
-                    real n = shfl_down1(eyh[0]);
-                    if (edge) n = s.Eo[1][m + VN];
-                    V hyj = V(0);
-                    hxj = ld<V>(s.Ho[0] + m);
-                    hzj = ld<V>(s.Ho[2] + m);
-                    h_cells<real, HX | HZ, false>(a, s, lx, yh, zc, j - 1, k, false, exh, eyh, ezh, V(0), eyhk, ex,
-                                                  ez, next_x<V>(eyh, n), V(0), hxj, hyj, hzj);
-                }
-            }
+            if (early && drow && xhalo) h_cell_yz(a, s, yc, zc, x0 - 1, j, k, c - 1, top, ez[0], ey[0], hyx, hzx);
             if (hrow) {
                 V exj, ezj;
                 if (last) {
@@ -800,7 +782,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
                 if (!first) {
                     hxj = rows.hx[p][w - 1][lane];
                     hzj = rows.hz[p][w - 1][lane];
-                } else if (!early) {
+                } else {
                     const size_t m = c - nxp;
                     const V exh = ld<V>(s.Eo[0] + m), ezh = ld<V>(s.Eo[2] + m);
                     if (!top) eyhk = ld<V>(s.Eo[1] + m + plane);
@@ -834,6 +816,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<real> &a, cYA<real> base,
             }
             if constexpr (EForm<real>::value) {
                 inject(a, x0, j, k, st, gx, gy, gz);
+                if (anyd) dx = ldnt<V>(a.Dx + c), dy = ldnt<V>(a.Dy + c), dz = ldnt<V>(a.Dz + c);
                 if (store) store_ef(a, s, c, mats, mid, fl, anyd, ex, ey, ez, dx, dy, dz, gx, gy, gz);
             } else {
                 inject(a, x0, j, k, st, dx, dy, dz);
@@ -1028,6 +1011,7 @@ void launch_yee(const YeeArgs<real> &a, hipStream_t s) {
         case 6: hipLaunchKernelGGL((yee_update<real, 6>), dim3(a.nblocks_pad), dim3(64, 6), lds, s, a); return;
         case 8: hipLaunchKernelGGL((yee_update<real, 8>), dim3(a.nblocks_pad), dim3(64, 8), lds, s, a); return;
         case 12: hipLaunchKernelGGL((yee_update<real, 12>), dim3(a.nblocks_pad), dim3(64, 12), lds, s, a); return;
+        case 16: hipLaunchKernelGGL((yee_update<real, 16>), dim3(a.nblocks_pad), dim3(64, 16), lds, s, a); return;
         default: hipLaunchKernelGGL((yee_update<real, 4>), dim3(a.nblocks_pad), dim3(64, 4), lds, s, a);
     }
 }

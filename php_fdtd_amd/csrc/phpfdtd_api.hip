@@ -152,7 +152,7 @@ struct phpfdtd_ctx {
     uint8_t *d_derive = nullptr;
     bool fused = true;     // one-pass Yee kernel (yee_update), else h_update + de_update
     int yee_kchunk = 16;   // z planes per Yee tile
-    int yee_rows = 12;     // y rows per work-group of the one-pass kernel (4, 6, 8 or 12)
+    int yee_rows = 8;      // y rows per work-group of the one-pass kernel (4, 6, 8, 12 or 16)
     // sources
     int nsrc = 0;
     int *d_src = nullptr;  // [kMaxSrc][4] (i, j, k, component)
@@ -344,14 +344,15 @@ static void plan_env(phpfdtd_ctx *x) {
         const int v = atoi(e);
         if (v >= 1) x->yee_kchunk = v;
     }
-    // one context, plan rebuilt per setting, 16-plane chunks: FP32 12 rows
-    // 11.49 ms against 11.77 (4 rows), 12.90 (8: one work-group per CU) and
-    // 15.31 (6: uneven over the 4 SIMDs); FP64 8 rows 21.70 ms against 22.44
-    // (4) and 23.25 (12: LDS-bound to one work-group per CU)
-    x->yee_rows = x->prec == PHPFDTD_FP32 ? 12 : 8;
+    // 16-plane chunks; FP32 (E-form, 127 VGPRs, 4 wavefronts per SIMD): 8
+    // rows, two work-groups per CU, 7.6-8.4 ms against 7.9-8.4 (16 rows, one
+    // work-group of 16 wavefronts) and 8.9-9.2 (12: one work-group of 12);
+    // FP64 8 rows 21.70 ms against 22.44 (4) and 23.25 (12: LDS-bound to one
+    // work-group per CU)
+    x->yee_rows = 8;
     if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
         const int v = atoi(e);
-        if (v == 4 || v == 6 || v == 8 || v == 12) x->yee_rows = v;
+        if (v == 4 || v == 6 || v == 8 || v == 12 || v == 16) x->yee_rows = v;
     }
 }
 

@@ -12,7 +12,7 @@ sys.path.insert(0, ROOT)
 
 import bench  # noqa: E402
 
-PLAN = {"fp32": ["yee_update[r12]", "ade_update[m5]", "ion_update[m4,N16]", "step_end"],
+PLAN = {"fp32": ["yee_update[r8]", "ade_update[m5]", "ion_update[m4,N16]", "step_end"],
         "fp64": ["yee_update[r8]", "ade_update[m5]", "ion_update[m4,N16]", "step_end"]}
 
 

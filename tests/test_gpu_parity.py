@@ -436,7 +436,7 @@ def test_fp32_ade_long_run():
 
 
 @pytest.mark.parametrize("case,rows", [("hbn", 4), ("aniso_loss", 4)] +
-                         [(c, r) for c in ("ragged", "tip_slab/8") for r in (4, 6, 8, 12)])
+                         [(c, r) for c in ("ragged", "tip_slab/8") for r in (4, 6, 8, 12, 16)])
 def test_fused_plan_matches_split(monkeypatch, case, rows):
     """The one-pass Yee kernel passes E and the new H between the rows of a
     work-group through LDS and recomputes the new H of the cells around the
