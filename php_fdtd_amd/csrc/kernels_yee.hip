@@ -985,6 +985,19 @@ __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a) {
 // Batched launches: the tiles of every domain of a batch in one grid.  The
 // XCD renumbering runs over the concatenated tiles.
 
+// One-pass kernel over the tiles of every domain of a batch (8-row
+// work-groups, as alone).  Each domain's argument block in the batch table
+// is read-only for the launch, so the plane loop re-reads it there.
+template <typename real>
+__global__ __launch_bounds__(64 * kBatchRows) void yee_update_batch(BatchLaunch<YeeArgs<real>> b) {
+    const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
+    if (g < 0) return;  // whole work-group
+    const int rid = (int)sld(b.map, g);
+    const YeeArgs<real> a = batch_args(b, rid);
+    const MatCoef<real> *mats = stage_mats(a);
+    yee_tile<real, kBatchRows>(a, (cYA<real>)(b.args + rid), g - sld(b.start, rid), mats);
+}
+
 template <typename real>
 __global__ __launch_bounds__(256) void h_update_batch(BatchLaunch<YeeArgs<real>> b) {
     const int g = xcd_tile(blockIdx.x, b.total, b.total_pad);
@@ -1028,6 +1041,13 @@ void launch_de(const YeeArgs<real> &a, hipStream_t s) {
 }
 
 template <typename real>
+void launch_yee_batch(const BatchLaunch<YeeArgs<real>> &b, int max_nmat, hipStream_t s) {
+    const size_t lds = (size_t)max_nmat * sizeof(MatCoef<real>);
+    if (b.total > 0)
+        hipLaunchKernelGGL((yee_update_batch<real>), dim3(b.total_pad), dim3(64, kBatchRows), lds, s, b);
+}
+
+template <typename real>
 void launch_h_batch(const BatchLaunch<YeeArgs<real>> &b, hipStream_t s) {
     if (b.total > 0) hipLaunchKernelGGL(h_update_batch<real>, dim3(b.total_pad), dim3(64, kBlockY), 0, s, b);
 }
@@ -1044,6 +1064,8 @@ template void launch_h<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_h<float>(const YeeArgs<float> &, hipStream_t);
 template void launch_de<double>(const YeeArgs<double> &, hipStream_t);
 template void launch_de<float>(const YeeArgs<float> &, hipStream_t);
+template void launch_yee_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);
+template void launch_yee_batch<float>(const BatchLaunch<YeeArgs<float>> &, int, hipStream_t);
 template void launch_h_batch<double>(const BatchLaunch<YeeArgs<double>> &, hipStream_t);
 template void launch_h_batch<float>(const BatchLaunch<YeeArgs<float>> &, hipStream_t);
 template void launch_de_batch<double>(const BatchLaunch<YeeArgs<double>> &, int, hipStream_t);

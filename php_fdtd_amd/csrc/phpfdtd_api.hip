@@ -29,6 +29,7 @@ namespace phpfdtd {
 template <typename real> void launch_yee(const YeeArgs<real> &, hipStream_t);
 template <typename real> void launch_h(const YeeArgs<real> &, hipStream_t);
 template <typename real> void launch_de(const YeeArgs<real> &, hipStream_t);
+template <typename real> void launch_yee_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
 template <typename real> void launch_h_batch(const BatchLaunch<YeeArgs<real>> &, hipStream_t);
 template <typename real> void launch_de_batch(const BatchLaunch<YeeArgs<real>> &, int, hipStream_t);
 template <typename real> void launch_slot_batch(const BatchLaunch<SlotArgs<real>> &, int, hipStream_t);
@@ -1669,6 +1670,7 @@ struct phpfdtd_batch {
     hipEvent_t ev = nullptr;
     std::vector<phpfdtd_ctx *> members;
     bool dirty = true;
+    bool fused = true;  // Yee plan of the members: one-pass (8-row work-groups) or split
     // device tables
     unsigned char *d_tab = nullptr;
     size_t d_cap = 0;
@@ -1834,8 +1836,12 @@ static void batch_step_t(phpfdtd_batch *b) {
         if constexpr (sizeof(real) == 8) return b->yee64;
         else return b->yee32;
     }();
-    launch_h_batch<real>(y, b->stream);
-    launch_de_batch<real>(y, b->max_nmat, b->stream);
+    if (b->fused) {
+        launch_yee_batch<real>(y, b->max_nmat, b->stream);
+    } else {
+        launch_h_batch<real>(y, b->stream);
+        launch_de_batch<real>(y, b->max_nmat, b->stream);
+    }
     for (auto &g : b->groups) {
         if constexpr (sizeof(real) == 8) launch_slot_batch<real>(g.b64, g.kind, b->stream);
         else launch_slot_batch<real>(g.b32, g.kind, b->stream);
@@ -1855,6 +1861,8 @@ static int batch_create_impl(int precision, phpfdtd_batch **out) {
     b->prec = precision == PHPFDTD_FP64 ? PHPFDTD_FP64 : PHPFDTD_FP32;
     b->form = precision;
     HIPCHK(hipGetDevice(&b->device));
+    const char *by = getenv("PHPFDTD_BATCH_YEE");
+    b->fused = !(by && strcmp(by, "split") == 0);
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
     for (int q = 0; q < 2; ++q) {
@@ -1875,13 +1883,13 @@ static int batch_add_impl(phpfdtd_batch *b, phpfdtd_ctx *x) {
     if (x->g.precision != b->form)
         return fail(PHPFDTD_EINVAL, "context precision %d differs from the batch's %d", x->g.precision, b->form);
     if (x->device != b->device) return fail(PHPFDTD_EINVAL, "context is on device %d, batch on %d", x->device, b->device);
-    // Batches run the split Yee plan (same results): on the serving mix
-    // (64 requests, 64^3 .. 256x256x160) it gives 22% more solves/s than the
-    // one-pass kernel, whose four-row work-groups and 16-plane chunks leave
-    // small domains with few tiles.  The member's own plan returns when it
-    // leaves the batch (its next rebuild).
-    x->fused = false;
-    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? 4 : 16;
+    // The batch's Yee plan (same results either way): the one-pass kernel
+    // with 8-row work-groups and 16-plane chunks, or the split pair with
+    // 4-plane FP32 chunks.  The member's own plan returns when it leaves the
+    // batch (its next rebuild).
+    x->fused = b->fused;
+    x->yee_rows = kBatchRows;
+    x->yee_kchunk = x->prec == PHPFDTD_FP32 && !b->fused ? 4 : 16;
     HIPCHK(hipSetDevice(x->device));
     if (!x->ev) HIPCHK(hipEventCreateWithFlags(&x->ev, hipEventDisableTiming));
     b->members.push_back(x);

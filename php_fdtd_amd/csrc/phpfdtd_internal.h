@@ -28,6 +28,7 @@ constexpr int kMaxSrc = 8;      // sources handled inside the D/E kernel
 constexpr int kMaxAtom = 32;    // lattice atoms per cell
 constexpr int kPackedKind = 64; // batch kind offset of compensated-state lattice kernels
 constexpr int kBlockY = 4;      // wavefronts per work-group of the split-plan Yee kernels (64 x 4 threads)
+constexpr int kBatchRows = 8;   // y rows (wavefronts) per work-group of the batched one-pass Yee kernel
 constexpr int kDiagWaves = 8192;  // wavefront partials of the one-pass lattice diagnostics
 constexpr double kEpsInf = 20.0;  // EPS_INF of the reference ADE (fdtd.c:202)
 

@@ -3,7 +3,8 @@
 Bar: a simulation computes exactly the same numbers in a batch as alone -
 every field component, the lattice / ADE state and the monitor trace are
 bit-identical (tolerance 0) in both forms, across membership changes and
-through the serving engine with admission queueing.
+through the serving engine with admission queueing, under both Yee plans
+of a batch (PHPFDTD_BATCH_YEE: the one-pass kernel, or the split pair).
 """
 import numpy as np
 import pytest
@@ -55,8 +56,10 @@ def alone(case, precision, steps):
     return snap
 
 
+@pytest.mark.parametrize("plan", ["fused", "split"])
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32, _abi.FP32_RV64])
-def test_batch_members_match_alone(precision):
+def test_batch_members_match_alone(monkeypatch, precision, plan):
+    monkeypatch.setenv("PHPFDTD_BATCH_YEE", plan)
     cases = [golden_cases.load(n) for n in CASES]
     T = 100
     ref = [alone(c, precision, T) for c in cases]
@@ -75,7 +78,9 @@ def test_batch_members_match_alone(precision):
     b.close()
 
 
-def test_batch_membership_changes():
+@pytest.mark.parametrize("plan", ["fused", "split"])
+def test_batch_membership_changes(monkeypatch, plan):
+    monkeypatch.setenv("PHPFDTD_BATCH_YEE", plan)
     prec = _abi.FP32
     ca, cb = golden_cases.load("hbn"), golden_cases.load("moo3")
     ra, rb = alone(ca, prec, 60), alone(cb, prec, 60)
