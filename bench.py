@@ -32,6 +32,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 PROFILE = os.path.join(ROOT, "profiles", "step_kernels_tip_slab_1024x1024x640.md")
+PROFILE_WORKLOAD = "tip_slab"  # the workload whose PMC traffic the profile holds
+GEOMETRY = {"lorentz_slab": "isotropic Lorentz slab", "lattice4_slab": "4-atom lattice slab",
+            "lattice16_slab": "16-atom lattice slab", "tip_slab": "16-atom lattice slab + Drude cone"}
 MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 METRIC = "Gcell-updates/s + HBM roofline fraction; solves/s, p50/p99, FP32 vs FP64"
@@ -304,7 +307,7 @@ def serving_batched(n, step_scale, seed=4):
     return dict(st, workload="serving mix, %d requests (seed %d), steps U(5000, 20000) x %g, FP32 form batched, "
                              "all queued at t=0" % (n, seed, step_scale),
                 cell_updates=float(sum(med.size * sp["grid"]["tsteps"] for sp, med in cases)),
-                step_model_bytes=float(sum(sp["grid"]["tsteps"] * workloads.step_bytes(sp, med, 4, 5)
+                step_model_bytes=float(sum(sp["grid"]["tsteps"] * workloads.step_bytes(sp, med, 4, 4.5)
                                            for sp, med in cases)),
                 mean_members=round(float(np.mean(members)), 2), max_members=int(max(members)))
 
@@ -528,7 +531,7 @@ def main():
     cells = float(med.size)
     res = {}
     precs = {"fp32": _abi.FP32, "fp32rv64": _abi.FP32_RV64, "fp64": _abi.FP64}
-    rv_bytes = {"fp32": 5, "fp32rv64": 8, "fp64": 8}
+    rv_bytes = {"fp32": 4.5, "fp32rv64": 8, "fp64": 8}
     for form in args.forms.split(","):
         prec = precs[form]
         log("form %s" % form)
@@ -546,7 +549,7 @@ def main():
     dom = max(r["prof"], key=lambda k: k["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     dom_sym = kernel_symbol(dom["name"], head)
-    traffic = profile_traffic(dom_sym)
+    traffic = profile_traffic(dom_sym) if args.workload == PROFILE_WORKLOAD else None
     sb = workloads.step_bytes(spec, med, word, rv_bytes[head])
     rv_desc = {"fp32": "compensated f32 (f64 integration)", "fp32rv64": "f64", "fp64": "f64"}[head]
     out = {
@@ -562,8 +565,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": "%s: 16-atom lattice slab + Drude cone, %s form (fields/CPML/ADE %s, lattice r,v "
-                               "%s)" % (spec["name"], head.upper(), "f64" if head == "fp64" else "f32", rv_desc),
+        "config": {"workload": "%s: %s, %s form (fields/CPML/ADE %s, lattice r,v %s)"
+                               % (spec["name"], GEOMETRY[args.workload], head.upper(),
+                                  "f64" if head == "fp64" else "f32", rv_desc),
                    "cells": int(cells), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -575,7 +579,7 @@ def main():
                           "frac": round(sb / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "model": ("C(18w+1) - 6w*plain cells (E-form: their D is not stored)" if head != "fp64"
                                     else "C(18w+1)") + " + 8w*CPML incidences + 9w*ADE cells + 12Nb*lattice cells, "
-                                   "b = %d bytes per stored r or v value" % rv_bytes[head]},
+                                   "b = %g bytes per stored r or v value" % rv_bytes[head]},
         "kernels_ms": {k["name"]: round(k["ms"], 4) for k in r["prof"]},
         "ms_per_step_with_sampling": round(r["sampled_ms"], 4),
     }

@@ -60,8 +60,8 @@ extern "C" {
 #define PHPFDTD_FP64 0  /* plain form: everything in FP64                          */
 #define PHPFDTD_FP32 1  /* precision form: fields, CPML, ADE in FP32; DMM.r on FP32
                            MFMA with FP32 accumulation; ion r,v integrated in FP64
-                           and stored as compensated FP32 (an FP32 word plus an
-                           8-bit compensation term: 32 significant bits)         */
+                           and stored as compensated FP32 (an FP32 word plus a
+                           4-bit compensation code: 28 significant bits)         */
 #define PHPFDTD_FP32_RV64 2  /* precision form with ion r,v stored in FP64      */
 /* Both precision forms update cells of plain (non-ADE, non-lattice)
  * materials in E-form: E^{n+1} = E^n + M.dD, the same update as the
@@ -278,7 +278,7 @@ int phpfdtd_write_field(phpfdtd_ctx *ctx, int comp, const double *in);
  * kind 0 = r, 1 = v (3*natom doubles per cell of material mat),
  *      2 = P_ADE, 3 = P_ADE old (3 doubles per cell of material mat).
  * In PHPFDTD_FP32, r and v are stored compensated: written values are
- * rounded to 32 significant bits (relative error <= 2^-31), and reading
+ * rounded to 28 significant bits (relative error <= 2^-27), and reading
  * returns the stored values exactly.
  * Cells of other materials are skipped: n_cells*width values are exchanged,
  * where n_cells is returned by phpfdtd_count_cells. */

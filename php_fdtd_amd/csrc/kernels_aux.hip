@@ -25,10 +25,8 @@ __device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool res
             double rv[2] = {0.0, 0.0};
             if (a.mon_ridx >= 0) {
                 if (a.mon_rc >= 0) {  // compensated FP32 state
-                    rv[0] = pk_unpack(static_cast<const float *>(a.mon_r)[a.mon_ridx],
-                                      static_cast<const int8_t *>(a.mon_r)[a.mon_rc]);
-                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx],
-                                      static_cast<const int8_t *>(a.mon_v)[a.mon_vc]);
+                    rv[0] = pk_unpack(static_cast<const float *>(a.mon_r)[a.mon_ridx], pk_code_at(a.mon_r, a.mon_rc));
+                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx], pk_code_at(a.mon_v, a.mon_vc));
                 } else {
                     rv[0] = static_cast<const double *>(a.mon_r)[a.mon_ridx];
                     rv[1] = static_cast<const double *>(a.mon_v)[a.mon_vidx];
@@ -172,23 +170,25 @@ __global__ __launch_bounds__(256) void diag_r(const void *r, const uint32_t *cel
         const int64_t ci = c % nxp, cj = (c / nxp) % ny, ck = c / ((int64_t)nxp * ny);
         const unsigned long long base = (unsigned long long)(((ci * ny + cj) * nz + ck) * 96);
         if (PK) {
-            // group g: words w[g][lane][0..3]; its compensation bytes are word
-            // g % 4 of lane's 16 bytes in compensation row g / 4
+            // group g: words w[g][lane][0..3]; its 4-bit codes are bits
+            // 16 (g % 2) .. +15 of dword (g % 8) / 2 of lane's 16 bytes in
+            // compensation row g / 8
             const float *tile = static_cast<const float *>(r) + t * lay.tile_r();
             const float4 *w = reinterpret_cast<const float4 *>(tile);
             const int4 *cb = reinterpret_cast<const int4 *>(tile + (int64_t)kPkWords * ng);
-            for (int g0 = 0; g0 < ng; g0 += 4) {
-                const int4 bv = cb[(g0 >> 2) * 64 + lane];
-                const int bws[4] = {bv.x, bv.y, bv.z, bv.w};
+            for (int g0 = 0; g0 < ng; g0 += 8) {
+                const int4 bv = cb[(g0 >> 3) * 64 + lane];
+                const uint32_t bws[4] = {(uint32_t)bv.x, (uint32_t)bv.y, (uint32_t)bv.z, (uint32_t)bv.w};
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
+                for (int h = 0; h < 8; ++h) {
                     const int g = g0 + h;
                     if (g >= ng) break;
                     const float4 hv = w[g * 64 + lane];
                     const float hw[4] = {hv.x, hv.y, hv.z, hv.w};
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        if (4 * g + e < n3) take(pk_unpack(hw[e], (int)(int8_t)(bws[h] >> (8 * e))), base + 4 * g + e);
+                        if (4 * g + e < n3)
+                            take(pk_unpack(hw[e], pk_code_of(bws[h / 2], 16 * (h % 2) + 4 * e)), base + 4 * g + e);
                 }
             }
         } else {

@@ -14,9 +14,9 @@
 // Lattice state layout (IonLayout): tiles of 64 slots, a lane owns a slot and
 // moves a group of its dofs as one 16-byte vector, so every wavefront access
 // is 1 KiB contiguous, streamed non-temporally: FP64 values in dof pairs, or
-// compensated FP32 (an FP32 word and an 8-bit correction per value) in dof
-// quads, the tile's correction bytes in 1 KiB rows after its word rows (four
-// groups' bytes per 16-byte lane vector).
+// compensated FP32 (an FP32 word and a 4-bit correction per value) in dof
+// quads, the tile's correction codes in 1 KiB rows after its word rows (eight
+// groups' codes per 16-byte lane vector).
 //
 // Plain form (FP64): one thread per slot, DMM r^n on the VALU in FP64 with
 // DMM/QeZV/QeZM as wavefront-uniform (scalar) operands.
@@ -250,8 +250,8 @@ __host__ __device__ constexpr int mfma_wave_lds(int na) {
 
 // Compensated FP32 state (IonLayout w = 4): `base` is the tile's first word;
 // group j holds the lanes' FP32 words at word 256 j + 4 lane, compensation
-// row b (after the ng groups) the lane's dwords of groups 4b .. 4b+3 at word
-// 256 (ng + b) + 4 lane.
+// row b (after the ng groups) the lane's 4-bit codes of groups 8b .. 8b+7 at
+// words 256 (ng + b) + 4 lane .. +3, two groups per dword.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ f32x4 ldquad(const uint32_t *base, int lane, int j) {
     return __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(base + (int64_t)j * kPkWords + 4 * lane));
@@ -266,11 +266,6 @@ __device__ __forceinline__ void stcomp(u32x4 v, uint32_t *base, int lane, int ng
     __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(base + (int64_t)(ng + b) * kPkWords + 4 * lane));
 }
 
-// Byte h of a packed dword as a signed value, and its replacement.
-__device__ __forceinline__ int byte_of(uint32_t w, int h) { return (int)(int8_t)(w >> (8 * h)); }
-__device__ __forceinline__ uint32_t with_byte(uint32_t w, int h, int b) {
-    return (w & ~(0xffu << (8 * h))) | ((uint32_t)(b & 0xff) << (8 * h));
-}
 
 template <int NA, int WPB, bool PK>
 __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const float *__restrict__ aop,
@@ -282,7 +277,7 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     constexpr int WL = mfma_wave_lds(NA);
     constexpr int W = PK ? 4 : 2;  // dofs per lane vector (IonLayout w)
     constexpr int NVX = (N3X + W - 1) / W;
-    constexpr int NBX = (NVX + 3) / 4;  // compensation rows (compensated state)
+    constexpr int NBX = (NVX + 7) / 8;  // compensation rows (compensated state)
     // compensated state up to 16 atoms: v is loaded with r, ahead of the MFMA
     // chain (register room; the runtime-N kernel loads it late)
     constexpr bool VPRE = PK && NVX <= 12;
@@ -290,7 +285,7 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     const int N3 = 3 * natom;
     const int NK = exact ? NKX : (N3 + 3) / 4, NMA = exact ? NAX : (N3 + 15) / 16;
     const int NV = (N3 + W - 1) / W;
-    const int NB = (NV + 3) / 4;
+    const int NB = (NV + 7) / 8;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t tile = blk * WPB + wave;
@@ -330,7 +325,7 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                     const uint32_t *vb = static_cast<const uint32_t *>(vv_) + tw;
                     const f32x4 g4 = ldquad(vb, lane, j);
                     for (int h = 0; h < 4; ++h) vw[4 * j + h] = g4[h];
-                    if (j % 4 == 0) vcr[j / 4] = ldcomp(vb, lane, NV, j / 4);
+                    if (j % 8 == 0) vcr[j / 8] = ldcomp(vb, lane, NV, j / 8);
                 }
             } else {
                 const f64x2 v2 = ldpair(static_cast<const double *>(rr_) + tb, j);
@@ -432,13 +427,15 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
             uint32_t *rb = static_cast<uint32_t *>(rr_) + tw, *vb = static_cast<uint32_t *>(vv_) + tw;
             f32x4 v4, r4;
             if constexpr (VPRE) {
-                if (j % 4 == 0) vcb = vcr[j / 4];
+                if (j % 8 == 0) vcb = vcr[j / 8];
                 v4 = (f32x4){vw[4 * j], vw[4 * j + 1], vw[4 * j + 2], vw[4 * j + 3]};
             } else {
-                if (j % 4 == 0) vcb = ldcomp(vb, lane, NV, j / 4);
+                if (j % 8 == 0) vcb = ldcomp(vb, lane, NV, j / 8);
                 v4 = ldquad(vb, lane, j);
             }
-            uint32_t vc = vcb[j % 4], rcn = rc[j / 4][j % 4];
+            // the codes of group j: bits 16 (j % 2) .. +15 of dword (j % 8) / 2
+            const int dw = (j % 8) / 2, sb = 16 * (j % 2);
+            uint32_t vc = vcb[dw], rcn = rc[j / 8][dw];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const int q = 4 * j + h;
@@ -448,23 +445,24 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                 const float af = buf[q * kRowA + lane] + sld(qm, at * 9 + 0 + d) * Ec[0] +
                                  sld(qm, at * 9 + 3 + d) * Ec[1] + sld(qm, at * 9 + 6 + d) * Ec[2];
                 const double am = (double)af;
-                const double rq = pk_unpack(rw[q], byte_of(rcn, h)), vq = pk_unpack(v4[h], byte_of(vc, h));
+                const double rq = pk_unpack(rw[q], pk_code_of(rcn, sb + 4 * h));
+                const double vq = pk_unpack(v4[h], pk_code_of(vc, sb + 4 * h));
                 float hw;
                 int lo;
                 pk_pack(rq + (vq * dt + am * hdt2), &hw, &lo);
                 r4[h] = hw;
-                rcn = with_byte(rcn, h, lo);
+                rcn = pk_with_code(rcn, sb + 4 * h, lo);
                 pk_pack(vq + am * dt, &hw, &lo);
                 v4[h] = hw;
-                vc = with_byte(vc, h, lo);
+                vc = pk_with_code(vc, sb + 4 * h, lo);
             }
             stquad(r4, rb, lane, j);
             stquad(v4, vb, lane, j);
-            rc[j / 4][j % 4] = rcn;
-            vcb[j % 4] = vc;
-            if (j % 4 == 3 || j == NV - 1) {  // row complete (padding dwords stored as loaded)
-                stcomp(rc[j / 4], rb, lane, NV, j / 4);
-                stcomp(vcb, vb, lane, NV, j / 4);
+            rc[j / 8][dw] = rcn;
+            vcb[dw] = vc;
+            if (j % 8 == 7 || j == NV - 1) {  // row complete (padding codes stored as loaded)
+                stcomp(rc[j / 8], rb, lane, NV, j / 8);
+                stcomp(vcb, vb, lane, NV, j / 8);
             }
         } else {
             double *rd = static_cast<double *>(rr_) + tb, *vd = static_cast<double *>(vv_) + tb;

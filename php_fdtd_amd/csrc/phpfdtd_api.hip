@@ -514,8 +514,8 @@ static void build_plan_t(phpfdtd_ctx *x) {
         if (sp.pre.natom > 0) {
             const double N3 = 3.0 * sp.pre.natom;
             snprintf(nm, sizeof(nm), "ion_update[m%d,N%d]", sp.id, sp.pre.natom);
-            // r, v read and written: 8 bytes per value, 5 for compensated FP32
-            k.bytes = n * (6 * w + 4 + 4 * N3 * (x->pk ? 5 : 8) + (sp.pre.ade ? 12 * w : 0));
+            // r, v read and written: 8 bytes per value, 4.5 for compensated FP32
+            k.bytes = n * (6 * w + 4 + 4 * N3 * (x->pk ? 4.5 : 8) + (sp.pre.ade ? 12 * w : 0));
             k.flops = n * (2 * N3 * N3 + 12 * N3 + 40);
             k.stage = K_ION;
             k.fn = [sa](hipStream_t s) { launch_ion<real>(sa, s); };
@@ -1377,10 +1377,11 @@ static int read_ions_impl(phpfdtd_ctx *x, int px, int py, int pz, double *r, dou
             const char *dev = (const char *)(k == 0 ? s.d_r : s.d_v);
             if (x->pk) {
                 float w;
-                int8_t b;
+                uint8_t b;
+                const int64_t ci = s.lay.c_index(slot, q);  // nibble index of the code
                 HIPCHK(hipMemcpy(&w, dev + at * 4, 4, hipMemcpyDeviceToHost));
-                HIPCHK(hipMemcpy(&b, dev + s.lay.c_index(slot, q), 1, hipMemcpyDeviceToHost));
-                out[q] = pk_unpack(w, b);
+                HIPCHK(hipMemcpy(&b, dev + (ci >> 1), 1, hipMemcpyDeviceToHost));
+                out[q] = pk_unpack(w, pk_code_at(&b, ci & 1));
             } else {
                 HIPCHK(hipMemcpy(out + q, dev + at * 8, 8, hipMemcpyDeviceToHost));
             }
@@ -1505,7 +1506,7 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     const size_t n = s.cells.size();
     int width;
     void *dev;
-    bool pk = false;  // compensated lattice state: FP32 words and compensation bytes in one array
+    bool pk = false;  // compensated lattice state: FP32 words and 4-bit compensation codes in one array
     size_t es;
     if (kind < 2) {
         if (!s.pre.natom) return fail(PHPFDTD_EINVAL, "material %d has no lattice", mat);
@@ -1531,16 +1532,16 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
             const size_t hi = tiled ? (size_t)(kind == 0 ? s.lay.r_index(slot, d) : s.lay.v_index(slot, d))
                                     : (size_t)d * n + slot;
             double &u = buf[q * width + d];
-            if (pk) {  // compensated FP32: written values are rounded to 32 significant bits
-                const size_t ci = (size_t)s.lay.c_index(slot, d);
+            if (pk) {  // compensated FP32: written values are rounded to 28 significant bits
+                const int64_t ci = s.lay.c_index(slot, d);  // nibble index of the code
                 float w;
                 int b;
                 if (write) {
                     pk_pack(u, &w, &b);
                     ((float *)h.data())[hi] = w;
-                    ((int8_t *)h.data())[ci] = (int8_t)b;
+                    pk_code_set(h.data(), ci, b);
                 } else {
-                    u = pk_unpack(((float *)h.data())[hi], ((int8_t *)h.data())[ci]);
+                    u = pk_unpack(((float *)h.data())[hi], pk_code_at(h.data(), ci));
                 }
             } else if (es == 8) {
                 if (write) ((double *)h.data())[hi] = u;

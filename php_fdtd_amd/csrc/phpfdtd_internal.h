@@ -134,11 +134,12 @@ inline int mfma_inst(int natom) {
 //   * FP64 state (plain form, and the precision form with FP64 r, v): w = 2,
 //     r[tile][dof / 2][64][2] doubles.
 //   * compensated FP32 state (precision form): w = 4; a tile is its ng groups
-//     of 1 KiB (the 64 lanes' four FP32 words), then ceil(ng / 4) blocks of
-//     1 KiB of compensation bytes: word 4 lane + h of block b holds the four
-//     compensation bytes of group 4b + h of that lane, so the bytes also move
-//     as one 16-byte vector per lane.  Words and bytes of an array stream from
-//     one address range.
+//     of 1 KiB (the 64 lanes' four FP32 words), then ceil(ng / 8) blocks of
+//     1 KiB of 4-bit compensation codes: word 4 lane + d of block b holds the
+//     codes of that lane's groups 8b + 2d (bits 0-15) and 8b + 2d + 1 (bits
+//     16-31), four bits per value, so the codes also move as one 16-byte
+//     vector per lane.  Words and codes of an array stream from one address
+//     range.
 constexpr int kPkWords = 256;  // 4-byte words per 1 KiB row
 
 struct IonLayout {
@@ -146,7 +147,7 @@ struct IonLayout {
     int w;  // dofs per lane vector: 2 (FP64) or 4 (compensated FP32)
     __host__ __device__ int n3p() const { return (n3 + w - 1) / w * w; }
     __host__ __device__ int ng() const { return n3p() / w; }    // dof groups (1 KiB rows) per tile
-    __host__ __device__ int nb() const { return (ng() + 3) / 4; }  // compensation rows (w = 4)
+    __host__ __device__ int nb() const { return (ng() + 7) / 8; }  // compensation rows (w = 4)
     // elements per tile: doubles (FP64), or 4-byte words (compensated)
     __host__ __device__ int64_t tile_r() const { return w == 4 ? (int64_t)kPkWords * (ng() + nb()) : 64LL * n3p(); }
     __host__ __device__ int64_t tile_v() const { return tile_r(); }
@@ -155,9 +156,13 @@ struct IonLayout {
         return (s >> 6) * tile_r() + (int64_t)(q / w) * 64 * w + (s & 63) * w + q % w;
     }
     __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
-    // byte offset of the compensation byte of dof q of slot s (compensated state)
+    // 4-bit code of dof q of slot s (compensated state), as a nibble index into
+    // the array (two per byte, the low one first): group j = q / 4 sits in
+    // dword (j % 8) / 2 of the lane's vector in row j / 8, bits 16 (j % 2) + 4 (q % 4)
     __host__ __device__ int64_t c_index(int64_t s, int q) const {
-        return ((s >> 6) * tile_r() + (int64_t)kPkWords * (ng() + q / 16) + (s & 63) * 4 + (q / 4) % 4) * 4 + q % 4;
+        const int j = q / 4;
+        const int64_t dw = (s >> 6) * tile_r() + (int64_t)kPkWords * (ng() + j / 8) + (s & 63) * 4 + (j % 8) / 2;
+        return dw * 8 + 4 * (j % 2) + q % 4;
     }
     // slot and dof of element idx; false for padding and compensation words
     __host__ __device__ bool r_decode(int64_t idx, int64_t nslot, int64_t *s, int *q) const {
@@ -178,11 +183,13 @@ inline IonLayout ion_layout(int natom, bool packed = false) {
     return l;
 }
 
-// Compensated FP32 value (hi, lo): x = hi + lo * 2^(e - 158), e the biased
-// exponent of hi clamped to >= 1, so lo counts units of 2^-8 of hi's ulp and
-// |lo| <= 127 covers the rounding error of hi = fl32(x).  The stored value
-// carries 32 significant bits; pack and unpack are exact in FP64 apart from
-// that final rounding of x - hi to a multiple of 2^-8 ulp.
+// Compensated FP32 value (hi, lo): x = hi + lo * 2^(e - 154), e the biased
+// exponent of hi clamped to >= 1, so lo counts units of 2^-4 of hi's ulp and
+// the 4-bit code lo in [-7, 7] covers the rounding error of hi = fl32(x)
+// (|x - hi| <= ulp / 2; a code of +-8 is clamped to +-7, so a stored value
+// stays strictly inside half an ulp of hi and re-packs to itself).  The
+// stored value carries 28 significant bits: pack and unpack are exact in FP64
+// apart from that final rounding of x - hi to a multiple of 2^-4 ulp.
 __host__ __device__ inline int pk_exp(float hi) {
     uint32_t b;
     __builtin_memcpy(&b, &hi, 4);
@@ -197,18 +204,34 @@ __host__ __device__ inline double pk_pow2(int k) {
     return d;
 }
 __host__ __device__ inline double pk_unpack(float hi, int lo) {
-    return fma((double)lo, pk_pow2(pk_exp(hi) - 158), (double)hi);  // exact
+    return fma((double)lo, pk_pow2(pk_exp(hi) - 154), (double)hi);  // exact
 }
 __host__ __device__ inline void pk_pack(double x, float *hi, int *lo) {
     const float h = (float)x;
-    // (x - h) in units of 2^-8 ulp(h), |.| <= 128, rounded to nearest even by
+    // (x - h) in units of 2^-4 ulp(h), |.| <= 8, rounded to nearest even by
     // adding 1.5 * 2^52: the integer lands in the low word of the sum
-    const double q = (x - (double)h) * pk_pow2(158 - pk_exp(h)) + 6755399441055744.0;
+    const double q = (x - (double)h) * pk_pow2(154 - pk_exp(h)) + 6755399441055744.0;
     uint64_t b;
     __builtin_memcpy(&b, &q, 8);
     const int n = (int)(uint32_t)b;
     *hi = h;
-    *lo = n < -127 ? -127 : n > 127 ? 127 : n;  // a non-finite x (diverged state) gives any code
+    *lo = n < -7 ? -7 : n > 7 ? 7 : n;  // a non-finite x (diverged state) gives any code
+}
+// The 4-bit code at nibble index n of an array (two per byte, the low first),
+// sign-extended; and its replacement.
+__host__ __device__ inline int pk_code_at(const void *base, int64_t n) {
+    const uint8_t b = static_cast<const uint8_t *>(base)[n >> 1];
+    const int v = (n & 1) ? (b >> 4) : (b & 15);
+    return (v ^ 8) - 8;
+}
+__host__ inline void pk_code_set(void *base, int64_t n, int c) {
+    uint8_t &b = static_cast<uint8_t *>(base)[n >> 1];
+    b = (n & 1) ? (uint8_t)((b & 0x0f) | ((c & 15) << 4)) : (uint8_t)((b & 0xf0) | (c & 15));
+}
+// The code at bit s of a dword of a compensation row, and its replacement.
+__host__ __device__ inline int pk_code_of(uint32_t w, int s) { return ((int)(w << (28 - s))) >> 28; }
+__host__ __device__ inline uint32_t pk_with_code(uint32_t w, int s, int c) {
+    return (w & ~(0xfu << s)) | ((uint32_t)(c & 15) << s);
 }
 
 // Lattice / ADE slot kernels: one material per launch.
@@ -257,7 +280,7 @@ struct StepEndArgs {
     const void *F[2][9];  // field arrays (real) of both E/H sets; the D entries are the same in both
     const void *mon_r, *mon_v;    // lattice state of the monitor cell's material
     int64_t mon_ridx, mon_vidx;   // elements of (slot, atom*3+dir) in r and v, -1 = no R/V
-    int64_t mon_rc, mon_vc;       // their compensation bytes (compensated state), else -1
+    int64_t mon_rc, mon_vc;       // nibble indices of their 4-bit codes (compensated state), else -1
     double dt;
     int part;                     // 0 whole row, 1 D and H, 2 t, E, R, V + step advance
     int mon_dfe;                  // E-form plain monitor cell: D = mon_dm . E (D is not stored)

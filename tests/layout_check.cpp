@@ -6,6 +6,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <string.h>
+
 #include <random>
 #include <set>
 #include <vector>
@@ -32,12 +34,12 @@ static void check_layout(int natom, bool packed) {
     CHECK(l.n3p() % w == 0 && l.n3p() >= n3 && l.n3p() < n3 + w, "natom %d", natom);
     CHECK(l.ng() * w == l.n3p(), "natom %d", natom);
     if (packed) {
-        // words of ng groups, then ceil(ng/4) rows of compensation bytes, 1 KiB each
-        CHECK(tile * es == 1024LL * (l.ng() + (l.ng() + 3) / 4), "natom %d tile %lld", natom, (long long)tile);
+        // words of ng groups, then ceil(ng/8) rows of 4-bit compensation codes, 1 KiB each
+        CHECK(tile * es == 1024LL * (l.ng() + (l.ng() + 7) / 8), "natom %d tile %lld", natom, (long long)tile);
     } else {
         CHECK(tile * es == 64LL * 8 * l.n3p(), "natom %d tile %lld", natom, (long long)tile);
     }
-    std::set<int64_t> words, bytes;
+    std::set<int64_t> words, codes;
     for (int t = 0; t < 2; ++t)
         for (int lane = 0; lane < 64; ++lane)
             for (int q = 0; q < l.n3p(); ++q) {
@@ -57,16 +59,18 @@ static void check_layout(int natom, bool packed) {
                                 (long long)s2, q2);
                 CHECK(!l.r_decode(ri, s, &s2, &q2), "natom %d: slot %lld past nslot decoded", natom, (long long)s);
                 if (packed) {
-                    const int64_t ci = l.c_index(s, q);
-                    const int64_t cw = ci / 4;
-                    // compensation bytes sit in the rows after the ng word rows of the same tile
+                    const int64_t ci = l.c_index(s, q);  // nibble index
+                    const int64_t cw = ci / 8;            // its dword
+                    // compensation codes sit in the rows after the ng word rows of the same tile
                     CHECK(cw >= t * tile + (int64_t)kPkWords * l.ng() && cw < (t + 1) * tile,
-                          "natom %d slot %lld dof %d byte %lld", natom, (long long)s, q, (long long)ci);
-                    // one 16-byte vector per lane and row: the lane's bytes of groups 4b .. 4b+3
-                    CHECK((cw - t * tile - (int64_t)kPkWords * l.ng()) * 4 == 1024LL * (q / 16) + 16LL * lane + 4 * ((q / 4) % 4) &&
-                              ci % 4 == q % 4,
+                          "natom %d slot %lld dof %d code %lld", natom, (long long)s, q, (long long)ci);
+                    // one 16-byte vector per lane and row: the lane's codes of groups 8b .. 8b+7, two
+                    // groups per dword, the group's four codes in one 16-bit half
+                    const int j = q / 4;
+                    CHECK((cw - t * tile - (int64_t)kPkWords * l.ng()) * 4 == 1024LL * (j / 8) + 16LL * lane + 4 * ((j % 8) / 2) &&
+                              ci % 8 == 4 * (j % 2) + q % 4,
                           "natom %d slot %lld dof %d", natom, (long long)s, q);
-                    CHECK(bytes.insert(ci).second, "natom %d: byte %lld used twice", natom, (long long)ci);
+                    CHECK(codes.insert(ci).second, "natom %d: code %lld used twice", natom, (long long)ci);
                 }
             }
     if (packed) {
@@ -76,7 +80,7 @@ static void check_layout(int natom, bool packed) {
             int q2;
             CHECK(!l.r_decode(e, 1 << 20, &s2, &q2), "natom %d: compensation element %lld decoded", natom, (long long)e);
         }
-        CHECK((int64_t)words.size() * 4 + (int64_t)bytes.size() <= 2 * tile * 4, "natom %d", natom);
+        CHECK((int64_t)words.size() * 8 + (int64_t)codes.size() <= 2 * tile * 8, "natom %d", natom);
     } else {
         CHECK((int64_t)words.size() == 2 * tile, "natom %d: FP64 tiles not covered", natom);
     }
@@ -94,18 +98,18 @@ static void check_values() {
     for (int n = 0; n < 200000; ++n) {
         const double x = ldexp(u(g), ex(g));
         pk_pack(x, &hi, &lo);
-        CHECK(lo >= -127 && lo <= 127, "x %.17g lo %d", x, lo);
+        CHECK(lo >= -7 && lo <= 7, "x %.17g lo %d", x, lo);
         CHECK(hi == (float)x, "x %.17g: word is not fl32(x)", x);
         const double y = pk_unpack(hi, lo);
-        // within half a unit (2^-8 ulp of the word) where the correction fits
-        // the byte, one unit where it is clamped to +-127
+        // within half a unit (2^-4 ulp of the word) where the correction fits
+        // the code, one unit where a +-8 is clamped to +-7
         const int e = pk_exp(hi);
-        const double unit = ldexp(1.0, e - 158);
-        const bool clamped = lo == 127 || lo == -127;
+        const double unit = ldexp(1.0, e - 154);
+        const bool clamped = lo == 7 || lo == -7;
         CHECK(fabs(y - x) <= (clamped ? 1.0 : 0.5) * unit, "x %.17g y %.17g unit %g lo %d", x, y, unit, lo);
-        // relative, normal words: 32 significant bits (2^-32), 2^-31 when clamped
+        // relative, normal words: 28 significant bits (2^-28), 2^-27 when clamped
         if (fabs(x) >= ldexp(1.0, -125))
-            CHECK(fabs(y - x) <= ldexp(fabs(x), clamped ? -31 : -32), "x %.17g y %.17g", x, y);
+            CHECK(fabs(y - x) <= ldexp(fabs(x), clamped ? -27 : -28), "x %.17g y %.17g", x, y);
         // idempotent: a stored value re-packs to the same code
         float hi2;
         int lo2;
@@ -113,11 +117,23 @@ static void check_values() {
         CHECK(hi2 == hi && lo2 == lo, "x %.17g: (%a, %d) -> (%a, %d)", x, hi, lo, hi2, lo2);
     }
     // just under the midpoint between 1 and its FP32 successor: the word is 1,
-    // the correction rounds to 128 and is clamped to 127, an error of 2^-31
+    // the correction rounds to 8 and is clamped to 7, an error of 2^-27
     const double x = 1.0 + ldexp(1.0, -24) - ldexp(1.0, -60);
     pk_pack(x, &hi, &lo);
-    CHECK(hi == 1.f && lo == 127, "clamp x %.17g: (%a, %d)", x, hi, lo);
-    CHECK(fabs(pk_unpack(hi, lo) - x) <= ldexp(fabs(x), -31), "clamp x %.17g", x);
+    CHECK(hi == 1.f && lo == 7, "clamp x %.17g: (%a, %d)", x, hi, lo);
+    CHECK(fabs(pk_unpack(hi, lo) - x) <= ldexp(fabs(x), -27), "clamp x %.17g", x);
+    // the codes at nibble indices: sign extension and replacement leave the neighbour alone
+    uint8_t buf[4] = {0, 0, 0, 0};
+    for (int c = -8; c <= 7; ++c)
+        for (int n = 0; n < 8; ++n) {
+            pk_code_set(buf, n, c);
+            CHECK(pk_code_at(buf, n) == c, "code %d at %d", c, n);
+            uint32_t dw;
+            memcpy(&dw, buf, 4);
+            CHECK(pk_code_of(dw, 4 * n) == c, "dword code %d at %d", c, n);
+            CHECK(pk_code_of(pk_with_code(dw, 4 * n, -c - 1), 4 * n) == -c - 1, "with_code %d at %d", c, n);
+            CHECK(pk_code_at(buf, n ^ 1) == pk_code_of(dw, 4 * (n ^ 1)), "neighbour of %d", n);
+        }
 }
 
 int main() {

@@ -281,10 +281,10 @@ def test_ragged_tiles_vs_oracle(monkeypatch, shape, L, plan):
 
 
 def test_compensated_state():
-    """Lattice r, v of the FP32 form are stored as an FP32 word plus an 8-bit
-    compensation term: any double written comes back within 2^-31 of itself
-    (2^-32 but for the clamped last code), values below the FP32 range within
-    2^-157, and storing what was read changes nothing."""
+    """Lattice r, v of the FP32 form are stored as an FP32 word plus a 4-bit
+    compensation code: any double written comes back within 2^-27 of itself
+    (2^-28 but for the clamped last codes), values below the FP32 range within
+    2^-153, and storing what was read changes nothing."""
     c = golden_cases.load("moo3")
     spec = c["spec"]
     sim = Simulation(spec, c["medium"], _abi.FP32)
@@ -296,8 +296,8 @@ def test_compensated_state():
         sim.set_state(4, kind, a)
         b = sim.state(4, kind)
         big = np.abs(a) >= 2.0 ** -126
-        assert np.all(np.abs(b[big] - a[big]) <= 2.0 ** -31 * np.abs(a[big])), kind
-        assert np.all(np.abs(b[~big] - a[~big]) <= 2.0 ** -157), kind
+        assert np.all(np.abs(b[big] - a[big]) <= 2.0 ** -27 * np.abs(a[big])), kind
+        assert np.all(np.abs(b[~big] - a[~big]) <= 2.0 ** -153), kind
         sim.set_state(4, kind, b)
         np.testing.assert_array_equal(sim.state(4, kind), b)
     sim.close()

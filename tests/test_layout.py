@@ -6,14 +6,17 @@ state I/O use:
   * for every atom count 1..32 and both state forms, each dof of each slot of
     two tiles has its own element inside its tile, a lane's group of dofs is
     one 16-byte vector and the 64 lanes' vectors one 1 KiB row;
-  * compensated form: each dof has its own compensation byte, in the rows
-    after the word rows of its tile, four groups per 16-byte lane vector; the
-    tile is 1 KiB per word row plus 1 KiB per four of them;
+  * compensated form: each dof has its own 4-bit compensation code, in the
+    rows after the word rows of its tile, eight groups per 16-byte lane
+    vector (two per dword, four codes per 16-bit half); the tile is 1 KiB
+    per word row plus 1 KiB per eight of them;
   * r_decode inverts r_index and rejects padding dofs, compensation rows and
     slots past the end;
   * pk: zero is exact; the word is fl32(x); a value comes back within half a
-    unit of 2^-8 ulp (2^-32 relative), one unit (2^-31) where the correction
-    is clamped to +-127; re-packing a stored value is the identity.
+    unit of 2^-4 ulp (2^-28 relative), one unit (2^-27) where a +-8
+    correction is clamped to +-7; re-packing a stored value is the identity;
+    a code read or written at a nibble index (or at a bit of a dword) is
+    sign-extended and leaves its neighbour as it was.
 """
 import os
 import subprocess
