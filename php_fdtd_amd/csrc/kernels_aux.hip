@@ -26,7 +26,7 @@ __device__ __forceinline__ void step_row(const StepEndArgs &a, bool dh, bool res
             if (a.mon_ridx >= 0) {
                 if (a.mon_rc >= 0) {  // compensated FP32 state
                     rv[0] = pk_unpack(static_cast<const float *>(a.mon_r)[a.mon_ridx], pk_code_at(a.mon_r, a.mon_rc));
-                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx], pk_code_at(a.mon_v, a.mon_vc));
+                    rv[1] = pk_unpack(static_cast<const float *>(a.mon_v)[a.mon_vidx], pk_code_at(a.mon_vcb, a.mon_vc));
                 } else {
                     rv[0] = static_cast<const double *>(a.mon_r)[a.mon_ridx];
                     rv[1] = static_cast<const double *>(a.mon_v)[a.mon_vidx];

@@ -286,6 +286,10 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     const int NK = exact ? NKX : (N3 + 3) / 4, NMA = exact ? NAX : (N3 + 15) / 16;
     const int NV = (N3 + W - 1) / W;
     const int NB = (NV + 7) / 8;
+    // compensated state: v's last groups may keep their codes in the free
+    // dwords 2 and 3 of r's last compensation row (IonLayout::shared)
+    const bool shr = PK && NV % 8 != 0 && NV % 8 <= 4;
+    const int NBV = NB - (shr ? 1 : 0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 4, col = lane & 15;
     const int64_t tile = blk * WPB + wave;
@@ -300,9 +304,9 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     }
     if (tile >= ntile) return;  // wave-uniform
     float *buf = lds_a + wave * WL;
-    // FP64 state: dof group j of this lane's slot at tb + 128 j; compensated: tile words at tw
+    // FP64 state: dof group j of this lane's slot at tb + 128 j; compensated: r's tile words at tw, v's at twv
     const int64_t tb = tile * 64 * W * (int64_t)NV + W * lane;
-    const int64_t tw = tile * (int64_t)kPkWords * (NV + NB);
+    const int64_t tw = tile * (int64_t)kPkWords * (NV + NB), twv = tile * (int64_t)kPkWords * (NV + NBV);
     // r^n: FP64 values (FP64 state), or FP32 words + compensation rows
     double r[PK ? 1 : W * NVX];
     float rw[PK ? W * NVX : 1];
@@ -322,10 +326,10 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
                 const f32x4 h4 = ldquad(rb, lane, j);
                 for (int h = 0; h < 4; ++h) rw[4 * j + h] = h4[h];
                 if constexpr (VPRE) {
-                    const uint32_t *vb = static_cast<const uint32_t *>(vv_) + tw;
+                    const uint32_t *vb = static_cast<const uint32_t *>(vv_) + twv;
                     const f32x4 g4 = ldquad(vb, lane, j);
                     for (int h = 0; h < 4; ++h) vw[4 * j + h] = g4[h];
-                    if (j % 8 == 0) vcr[j / 8] = ldcomp(vb, lane, NV, j / 8);
+                    if (j % 8 == 0 && j / 8 < NBV) vcr[j / 8] = ldcomp(vb, lane, NV, j / 8);
                 }
             } else {
                 const f64x2 v2 = ldpair(static_cast<const double *>(rr_) + tb, j);
@@ -424,18 +428,20 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
     for (int j = 0; j < NVX; ++j) {
         if (j >= NV) break;
         if constexpr (PK) {
-            uint32_t *rb = static_cast<uint32_t *>(rr_) + tw, *vb = static_cast<uint32_t *>(vv_) + tw;
+            uint32_t *rb = static_cast<uint32_t *>(rr_) + tw, *vb = static_cast<uint32_t *>(vv_) + twv;
+            const bool vin_r = shr && j / 8 == NB - 1;  // v's codes of group j in r's row
             f32x4 v4, r4;
             if constexpr (VPRE) {
-                if (j % 8 == 0) vcb = vcr[j / 8];
+                if (j % 8 == 0 && !vin_r) vcb = vcr[j / 8];
                 v4 = (f32x4){vw[4 * j], vw[4 * j + 1], vw[4 * j + 2], vw[4 * j + 3]};
             } else {
-                if (j % 8 == 0) vcb = ldcomp(vb, lane, NV, j / 8);
+                if (j % 8 == 0 && !vin_r) vcb = ldcomp(vb, lane, NV, j / 8);
                 v4 = ldquad(vb, lane, j);
             }
             // the codes of group j: bits 16 (j % 2) .. +15 of dword (j % 8) / 2
+            // (v's in dword 2 + (j % 8) / 2 of r's row when shared)
             const int dw = (j % 8) / 2, sb = 16 * (j % 2);
-            uint32_t vc = vcb[dw], rcn = rc[j / 8][dw];
+            uint32_t vc = vin_r ? rc[j / 8][2 + dw] : vcb[dw], rcn = rc[j / 8][dw];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const int q = 4 * j + h;
@@ -459,10 +465,11 @@ __device__ __forceinline__ void ion_mfma_block(const SlotArgs<float> &a, const f
             stquad(r4, rb, lane, j);
             stquad(v4, vb, lane, j);
             rc[j / 8][dw] = rcn;
-            vcb[dw] = vc;
+            if (vin_r) rc[j / 8][2 + dw] = vc;
+            else vcb[dw] = vc;
             if (j % 8 == 7 || j == NV - 1) {  // row complete (padding codes stored as loaded)
                 stcomp(rc[j / 8], rb, lane, NV, j / 8);
-                stcomp(vcb, vb, lane, NV, j / 8);
+                if (!vin_r) stcomp(vcb, vb, lane, NV, j / 8);
             }
         } else {
             double *rd = static_cast<double *>(rr_) + tb, *vd = static_cast<double *>(vv_) + tb;

@@ -165,7 +165,7 @@ struct phpfdtd_ctx {
     int64_t ring_rows = 0, mon_read = 0;
     bool mon_on = false;
     int64_t mon_c = -1, mon_ridx = -1, mon_vidx = -1;
-    const void *mon_r = nullptr, *mon_v = nullptr;
+    const void *mon_r = nullptr, *mon_v = nullptr, *mon_vcb = nullptr;
     int64_t mon_rc = -1, mon_vc = -1;
     unsigned long long *d_diag = nullptr;
     unsigned long long *d_diag_part = nullptr;  // [kDiagWaves][2] wavefront pairs of the lattice diagnostics
@@ -566,6 +566,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
     se.mon_v = x->mon_v;
     se.mon_rc = x->mon_rc;
     se.mon_vc = x->mon_vc;
+    se.mon_vcb = x->mon_vcb;
     se.mon_ridx = x->mon_ridx;
     se.mon_vidx = x->mon_vidx;
     se.dt = x->g.dt;
@@ -960,7 +961,7 @@ static int set_monitor_impl(phpfdtd_ctx *x, int mx, int my, int mz, int atom, in
     x->mon_on = true;
     x->mon_c = zf < x->nz ? (int64_t)x->fidx(mx, my, zf) : -1;
     x->mon_ridx = x->mon_vidx = -1;
-    x->mon_r = x->mon_v = nullptr;
+    x->mon_r = x->mon_v = x->mon_vcb = nullptr;
     x->mon_rc = x->mon_vc = -1;
     const int m = x->medium[x->fidx(mx, my, mz)];
     const int sp = x->special_of[m];
@@ -972,8 +973,10 @@ static int set_monitor_impl(phpfdtd_ctx *x, int mx, int my, int mz, int atom, in
         x->mon_r = x->specials[sp].d_r;
         x->mon_v = x->specials[sp].d_v;
         if (x->pk) {
+            bool in_r = false;
             x->mon_rc = x->specials[sp].lay.c_index(slot, atom * 3 + dir);
-            x->mon_vc = x->mon_rc;
+            x->mon_vc = x->specials[sp].lay.cv_index(slot, atom * 3 + dir, &in_r);
+            x->mon_vcb = in_r ? x->mon_r : x->mon_v;
         }
     }
     touch(x);
@@ -1378,9 +1381,10 @@ static int read_ions_impl(phpfdtd_ctx *x, int px, int py, int pz, double *r, dou
             if (x->pk) {
                 float w;
                 uint8_t b;
-                const int64_t ci = s.lay.c_index(slot, q);  // nibble index of the code
+                bool in_r = k == 0;  // v's code may sit in r's array
+                const int64_t ci = k == 0 ? s.lay.c_index(slot, q) : s.lay.cv_index(slot, q, &in_r);  // nibble index
                 HIPCHK(hipMemcpy(&w, dev + at * 4, 4, hipMemcpyDeviceToHost));
-                HIPCHK(hipMemcpy(&b, dev + (ci >> 1), 1, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&b, (const char *)(in_r ? s.d_r : s.d_v) + (ci >> 1), 1, hipMemcpyDeviceToHost));
                 out[q] = pk_unpack(w, pk_code_at(&b, ci & 1));
             } else {
                 HIPCHK(hipMemcpy(out + q, dev + at * 8, 8, hipMemcpyDeviceToHost));
@@ -1526,6 +1530,10 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
     std::vector<unsigned char> h(nelem * es);
     const std::vector<int64_t> ord = file_order(x, s);
     if (!write || tiled) HIPCHK(hipMemcpy(h.data(), dev, h.size(), hipMemcpyDeviceToHost));
+    // v's last codes in r's array (IonLayout::shared): r's array comes along
+    const bool vshare = pk && kind == 1 && s.lay.shared();
+    std::vector<unsigned char> hr(vshare ? (n + 63) / 64 * (size_t)s.lay.tile_r() * es : 0);
+    if (vshare) HIPCHK(hipMemcpy(hr.data(), s.d_r, hr.size(), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < n; ++q) {
         const int64_t slot = ord[q];
         for (int d = 0; d < width; ++d) {
@@ -1533,15 +1541,17 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
                                     : (size_t)d * n + slot;
             double &u = buf[q * width + d];
             if (pk) {  // compensated FP32: written values are rounded to 28 significant bits
-                const int64_t ci = s.lay.c_index(slot, d);  // nibble index of the code
+                bool in_r = kind == 0;
+                const int64_t ci = kind == 0 ? s.lay.c_index(slot, d) : s.lay.cv_index(slot, d, &in_r);  // nibble index
+                unsigned char *cb = kind == 1 && in_r ? hr.data() : h.data();
                 float w;
                 int b;
                 if (write) {
                     pk_pack(u, &w, &b);
                     ((float *)h.data())[hi] = w;
-                    pk_code_set(h.data(), ci, b);
+                    pk_code_set(cb, ci, b);
                 } else {
-                    u = pk_unpack(((float *)h.data())[hi], pk_code_at(h.data(), ci));
+                    u = pk_unpack(((float *)h.data())[hi], pk_code_at(cb, ci));
                 }
             } else if (es == 8) {
                 if (write) ((double *)h.data())[hi] = u;
@@ -1553,6 +1563,7 @@ static int state_io(phpfdtd_ctx *x, int mat, int kind, double *buf, int write) {
         }
     }
     if (write) HIPCHK(hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice));
+    if (write && vshare) HIPCHK(hipMemcpy(s.d_r, hr.data(), hr.size(), hipMemcpyHostToDevice));
     return 0;
 }
 

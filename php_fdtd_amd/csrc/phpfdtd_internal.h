@@ -139,7 +139,11 @@ inline int mfma_inst(int natom) {
 //     codes of that lane's groups 8b + 2d (bits 0-15) and 8b + 2d + 1 (bits
 //     16-31), four bits per value, so the codes also move as one 16-byte
 //     vector per lane.  Words and codes of an array stream from one address
-//     range.
+//     range.  When the last block of r holds 1-4 groups (ng % 8 in 1..4: 16
+//     atoms, 4 atoms ...) its words 2 and 3 of each lane are free and hold
+//     the codes of v's last groups instead, so v's tile has one block fewer
+//     (shared(); r and v are advanced together, so the block is loaded and
+//     stored once per step).
 constexpr int kPkWords = 256;  // 4-byte words per 1 KiB row
 
 struct IonLayout {
@@ -147,21 +151,35 @@ struct IonLayout {
     int w;  // dofs per lane vector: 2 (FP64) or 4 (compensated FP32)
     __host__ __device__ int n3p() const { return (n3 + w - 1) / w * w; }
     __host__ __device__ int ng() const { return n3p() / w; }    // dof groups (1 KiB rows) per tile
-    __host__ __device__ int nb() const { return (ng() + 7) / 8; }  // compensation rows (w = 4)
+    __host__ __device__ int nb() const { return (ng() + 7) / 8; }  // compensation rows of r (w = 4)
+    // v's last groups keep their codes in the free half of r's last row
+    __host__ __device__ bool shared() const { return w == 4 && ng() % 8 != 0 && ng() % 8 <= 4; }
+    __host__ __device__ int nbv() const { return nb() - (shared() ? 1 : 0); }  // compensation rows of v
     // elements per tile: doubles (FP64), or 4-byte words (compensated)
     __host__ __device__ int64_t tile_r() const { return w == 4 ? (int64_t)kPkWords * (ng() + nb()) : 64LL * n3p(); }
-    __host__ __device__ int64_t tile_v() const { return tile_r(); }
+    __host__ __device__ int64_t tile_v() const { return w == 4 ? (int64_t)kPkWords * (ng() + nbv()) : 64LL * n3p(); }
     // element of dof q of slot s: the double, or the FP32 word
     __host__ __device__ int64_t r_index(int64_t s, int q) const {
         return (s >> 6) * tile_r() + (int64_t)(q / w) * 64 * w + (s & 63) * w + q % w;
     }
-    __host__ __device__ int64_t v_index(int64_t s, int q) const { return r_index(s, q); }
+    __host__ __device__ int64_t v_index(int64_t s, int q) const {
+        return (s >> 6) * tile_v() + (int64_t)(q / w) * 64 * w + (s & 63) * w + q % w;
+    }
     // 4-bit code of dof q of slot s (compensated state), as a nibble index into
-    // the array (two per byte, the low one first): group j = q / 4 sits in
+    // r's array (two per byte, the low one first): group j = q / 4 sits in
     // dword (j % 8) / 2 of the lane's vector in row j / 8, bits 16 (j % 2) + 4 (q % 4)
     __host__ __device__ int64_t c_index(int64_t s, int q) const {
         const int j = q / 4;
         const int64_t dw = (s >> 6) * tile_r() + (int64_t)kPkWords * (ng() + j / 8) + (s & 63) * 4 + (j % 8) / 2;
+        return dw * 8 + 4 * (j % 2) + q % 4;
+    }
+    // the same for v: a nibble index into v's array, or, for the groups of a
+    // shared last row (*in_r set), into r's array at dwords 2 + (j % 8) / 2
+    __host__ __device__ int64_t cv_index(int64_t s, int q, bool *in_r) const {
+        const int j = q / 4;
+        *in_r = shared() && j / 8 == nb() - 1;
+        const int64_t dw = *in_r ? (s >> 6) * tile_r() + (int64_t)kPkWords * (ng() + j / 8) + (s & 63) * 4 + 2 + (j % 8) / 2
+                                 : (s >> 6) * tile_v() + (int64_t)kPkWords * (ng() + j / 8) + (s & 63) * 4 + (j % 8) / 2;
         return dw * 8 + 4 * (j % 2) + q % 4;
     }
     // slot and dof of element idx; false for padding and compensation words
@@ -281,6 +299,7 @@ struct StepEndArgs {
     const void *mon_r, *mon_v;    // lattice state of the monitor cell's material
     int64_t mon_ridx, mon_vidx;   // elements of (slot, atom*3+dir) in r and v, -1 = no R/V
     int64_t mon_rc, mon_vc;       // nibble indices of their 4-bit codes (compensated state), else -1
+    const void *mon_vcb;          // array holding v's code: v, or r for a shared last row
     double dt;
     int part;                     // 0 whole row, 1 D and H, 2 t, E, R, V + step advance
     int mon_dfe;                  // E-form plain monitor cell: D = mon_dm . E (D is not stored)

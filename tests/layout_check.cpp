@@ -30,24 +30,33 @@ static void check_layout(int natom, bool packed) {
     const IonLayout l = ion_layout(natom, packed);
     const int n3 = 3 * natom, w = packed ? 4 : 2;
     const int64_t es = packed ? 4 : 8;  // bytes per element
-    const int64_t tile = l.tile_r();
+    const int64_t tile = l.tile_r(), tilev = l.tile_v();
     CHECK(l.n3p() % w == 0 && l.n3p() >= n3 && l.n3p() < n3 + w, "natom %d", natom);
     CHECK(l.ng() * w == l.n3p(), "natom %d", natom);
+    // v's last codes share r's last compensation row exactly when that row holds 1-4 groups
+    const int tail = l.ng() % 8;
+    const bool shared = packed && tail >= 1 && tail <= 4;
+    CHECK(l.shared() == shared, "natom %d", natom);
     if (packed) {
-        // words of ng groups, then ceil(ng/8) rows of 4-bit compensation codes, 1 KiB each
-        CHECK(tile * es == 1024LL * (l.ng() + (l.ng() + 7) / 8), "natom %d tile %lld", natom, (long long)tile);
+        // words of ng groups, then ceil(ng/8) rows of 4-bit compensation codes, 1 KiB each;
+        // v's tile has one row fewer when shared
+        const int nb = (l.ng() + 7) / 8;
+        CHECK(tile * es == 1024LL * (l.ng() + nb), "natom %d tile %lld", natom, (long long)tile);
+        CHECK(tilev * es == 1024LL * (l.ng() + nb - (shared ? 1 : 0)), "natom %d v tile %lld", natom, (long long)tilev);
     } else {
-        CHECK(tile * es == 64LL * 8 * l.n3p(), "natom %d tile %lld", natom, (long long)tile);
+        CHECK(tile * es == 64LL * 8 * l.n3p() && tilev == tile, "natom %d tile %lld", natom, (long long)tile);
     }
-    std::set<int64_t> words, codes;
+    std::set<int64_t> words, codes, vwords, vcodes;
     for (int t = 0; t < 2; ++t)
         for (int lane = 0; lane < 64; ++lane)
             for (int q = 0; q < l.n3p(); ++q) {
                 const int64_t s = 64LL * t + lane;
                 const int64_t ri = l.r_index(s, q);
                 CHECK(ri >= t * tile && ri < (t + 1) * tile, "natom %d slot %lld dof %d", natom, (long long)s, q);
-                CHECK(l.v_index(s, q) == ri, "natom %d", natom);
+                const int64_t vi = l.v_index(s, q);
+                CHECK(vi - t * tilev == ri - t * tile, "natom %d", natom);  // same place in the tile
                 CHECK(words.insert(ri).second, "natom %d: element %lld used twice", natom, (long long)ri);
+                CHECK(vwords.insert(vi).second, "natom %d: v element %lld used twice", natom, (long long)vi);
                 // a lane's w dofs of one group are one 16-byte vector, the 64 lanes' vectors one 1 KiB row
                 CHECK((ri - t * tile) * es == 1024LL * (q / w) + 16LL * lane + es * (q % w), "natom %d slot %lld dof %d",
                       natom, (long long)s, q);
@@ -71,6 +80,20 @@ static void check_layout(int natom, bool packed) {
                               ci % 8 == 4 * (j % 2) + q % 4,
                           "natom %d slot %lld dof %d", natom, (long long)s, q);
                     CHECK(codes.insert(ci).second, "natom %d: code %lld used twice", natom, (long long)ci);
+                    // v's code: the same place in v's tile, or, for a group of r's shared last row,
+                    // the same bits of dword 2 + (j % 8) / 2 of that row in r's tile
+                    bool in_r = false;
+                    const int64_t cv = l.cv_index(s, q, &in_r);
+                    CHECK(in_r == (shared && j / 8 == (l.ng() + 7) / 8 - 1), "natom %d dof %d", natom, q);
+                    if (in_r) {
+                        CHECK(cv == ci + 16, "natom %d slot %lld dof %d", natom, (long long)s, q);
+                        CHECK(codes.insert(cv).second, "natom %d: v code %lld on an r code", natom, (long long)cv);
+                    } else {
+                        CHECK(cv - t * tilev * 8 == ci - t * tile * 8, "natom %d slot %lld dof %d", natom, (long long)s, q);
+                        CHECK(cv / 8 >= t * tilev + (int64_t)kPkWords * l.ng() && cv / 8 < (t + 1) * tilev,
+                              "natom %d slot %lld dof %d v code %lld", natom, (long long)s, q, (long long)cv);
+                        CHECK(vcodes.insert(cv).second, "natom %d: v code %lld used twice", natom, (long long)cv);
+                    }
                 }
             }
     if (packed) {

@@ -10,6 +10,10 @@ state I/O use:
     rows after the word rows of its tile, eight groups per 16-byte lane
     vector (two per dword, four codes per 16-bit half); the tile is 1 KiB
     per word row plus 1 KiB per eight of them;
+  * when r's last code row holds 1-4 groups (and only then) v's codes of
+    those groups sit in its dwords 2 and 3, next to r's codes of the same
+    groups, on no other code, and v's tile is one row shorter; v's words and
+    other codes sit where r's sit in r's tile;
   * r_decode inverts r_index and rejects padding dofs, compensation rows and
     slots past the end;
   * pk: zero is exact; the word is fl32(x); a value comes back within half a
