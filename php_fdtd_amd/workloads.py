@@ -339,7 +339,7 @@ def step_bytes(spec, med, word, rv_bytes=8, eform=None):
     """Algorithmic HBM bytes of one time step: each state word read and written
     once.  C (18w + 1) + 8w * CPML incidences + 9w * ADE cells
     + 12 N b * lattice cells, b the bytes per stored r or v value (8 for FP64,
-    5 for compensated FP32): 96 N or 60 N.  In E-form (the FP32 forms; the
+    4.5 for compensated FP32: a word and a 4-bit code): 96 N or 54 N.  In E-form (the FP32 forms; the
     default for w = 4) plain cells store E and H alone: 12w + 1 instead of
     18w + 1 (their D is not stored)."""
     if eform is None:

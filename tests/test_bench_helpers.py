@@ -3,7 +3,7 @@ maps to a device symbol that has rows in the kernel trace and the PMC traffic
 table of profiles/step_kernels_tip_slab_1024x1024x640.md, and the traffic it
 reads back for the lattice kernel equals the bytes its state layout moves
 (FP64: the compulsory bytes; compensated FP32: the words and the 1 KiB rows
-of 4-bit codes, the last of which a 16-atom tile fills half)."""
+of 4-bit codes, v's last codes in the free half of r's last row)."""
 import os
 import re
 import sys
@@ -35,9 +35,10 @@ def test_plan_kernels_have_profile_rows():
 
 def test_lattice_kernel_traffic_is_its_compulsory_bytes():
     # 50,331,648 lattice slots: 6w + 4 bytes each (D, E, cell index), and r and v
-    # read and written in 64-slot tiles of 12 word rows + 2 code rows of 1 KiB (N = 16)
+    # read and written in 64-slot tiles of 1 KiB rows (N = 16): r 12 word rows + 2
+    # code rows, v 12 + 1
     slots = 1024 * 1024 * 48
-    moved = slots * (6 * 4 + 4) + slots // 64 * 4 * 1024 * (12 + 2)
+    moved = slots * (6 * 4 + 4) + slots // 64 * 2 * 1024 * ((12 + 2) + (12 + 1))
     t = bench.profile_traffic(bench.kernel_symbol("ion_update[m4,N16]", "fp32"))
     assert t is not None and abs(t / moved - 1) < 0.002, (t, moved)
     t64 = bench.profile_traffic(bench.kernel_symbol("ion_update[m4,N16]", "fp64"))
