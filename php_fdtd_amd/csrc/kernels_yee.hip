@@ -6,8 +6,11 @@
 // stream moves in 16-byte lane accesses.  A wavefront marches a short chunk
 // of z planes, carrying the z neighbours in registers; x neighbours are the
 // adjacent element or a cross-lane shuffle.  Hardware places work-group b on
-// XCD (b mod 8); xcd_tile() renumbers the groups so each XCD owns one
-// contiguous range of tiles and neighbouring tiles share its L2.
+// XCD (b mod 8); band_tile() renumbers the groups of one domain so that the
+// x segments of a band of rows run on one XCD (sharing its L2 for the x halo
+// cells) and consecutive bands go to the eight XCDs in turn, which keeps all
+// of them streaming the same region of each plane.  Batches use xcd_tile(),
+// one contiguous range of tiles per XCD.
 //
 // E and H are double-buffered (YeeArgs): step s reads set s & 1 and writes
 // set (s + 1) & 1.  D and the D psi are updated in place.
@@ -49,6 +52,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks, int nblocks_pad) {
     const int t = (b & 7) * per + (b >> 3);
     return t < nblocks ? t : -1;
 }
+
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -962,14 +966,14 @@ __device__ __forceinline__ MatCoef<real> *stage_mats(const YeeArgs<real> &a) {
 template <typename real, int ROWS>
 __global__ __launch_bounds__(64 * ROWS) void yee_update(YeeArgs<real> a) {
     const MatCoef<real> *mats = stage_mats(a);
-    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
+    const int t = band_tile(blockIdx.x, a.nblocks, a.nbx);
     if (t < 0) return;
     yee_tile<real, ROWS>(a, (cYA<real>)__builtin_amdgcn_kernarg_segment_ptr(), t, mats);
 }
 
 template <typename real>
 __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
-    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
+    const int t = band_tile(blockIdx.x, a.nblocks, a.nbx);
     if (t < 0) return;
     h_tile<real>(a, t);
 }
@@ -977,7 +981,7 @@ __global__ __launch_bounds__(256) void h_update(YeeArgs<real> a) {
 template <typename real>
 __global__ __launch_bounds__(256) void de_update(YeeArgs<real> a) {
     const MatCoef<real> *mats = stage_mats(a);
-    const int t = xcd_tile(blockIdx.x, a.nblocks, a.nblocks_pad);
+    const int t = band_tile(blockIdx.x, a.nblocks, a.nbx);
     if (t < 0) return;
     de_tile<real>(a, t, mats);
 }

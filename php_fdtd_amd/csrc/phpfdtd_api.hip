@@ -427,7 +427,7 @@ static void build_plan_t(phpfdtd_ctx *x) {
     a.kend = x->nz;
     a.nbz = (x->nz + kc - 1) / kc;
     a.nblocks = a.nbx * a.nby * a.nbz;
-    a.nblocks_pad = (a.nblocks + 7) / 8 * 8;
+    a.nblocks_pad = (a.nblocks + 8 * a.nbx - 1) / (8 * a.nbx) * (8 * a.nbx);  // whole groups of band_tile
     a.nsrc = x->nsrc;
     a.src = x->d_src;
     a.src_val = (const real *)x->d_src_val;

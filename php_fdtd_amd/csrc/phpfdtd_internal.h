@@ -118,6 +118,19 @@ struct YeeArgs {
 __host__ __device__ inline int mfma_nk(int na) { return (3 * na + 3) / 4; }
 __host__ __device__ inline int mfma_nma(int na) { return (3 * na + 15) / 16; }
 
+// Work-group b of a single-domain Yee launch -> its tile: groups of g
+// consecutive tiles (g = the x segments of a band of rows; tiles are numbered
+// x segment fastest) are dealt to the XCDs in turn (work-group b runs on XCD
+// b mod 8).  The launch runs a multiple of 8 g work-groups, so every tile
+// has one; the rest get -1.  Measured on the 1024x1024x640 FP32 step:
+// 7.71-7.73 ms against 8.35-8.41 ms for one contiguous range of tiles per
+// XCD, 8.21-8.28 ms for tile b, 8.08-8.12 ms for four bands per group.
+__host__ __device__ inline int band_tile(int b, int nblocks, int g) {
+    const int x = b & 7, i = b >> 3;
+    const int t = ((i / g) * 8 + x) * g + i % g;
+    return t < nblocks ? t : -1;
+}
+
 // natom of the lattice kernel instantiation (register arrays sized for it;
 // the generic one runs to the material's own counts).
 inline int mfma_inst(int natom) {

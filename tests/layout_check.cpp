@@ -109,6 +109,26 @@ static void check_layout(int natom, bool packed) {
     }
 }
 
+// band_tile: over a launch of a multiple of 8 g work-groups every tile has
+// exactly one work-group, and the g tiles of a group run on one XCD, the
+// groups going to the XCDs in turn
+static void check_band_tile() {
+    for (int g = 1; g <= 9; ++g)
+        for (int n = 1; n <= 700; n += (n < 40 ? 1 : 37)) {
+            const int pad = (n + 8 * g - 1) / (8 * g) * (8 * g);
+            std::vector<int> seen(n, -1);
+            for (int b = 0; b < pad; ++b) {
+                const int t = band_tile(b, n, g);
+                CHECK(t >= -1 && t < n, "g %d n %d b %d -> %d", g, n, b, t);
+                if (t < 0) continue;
+                CHECK(seen[t] < 0, "g %d n %d: tile %d twice", g, n, t);
+                seen[t] = b;
+                CHECK(b % 8 == (t / g) % 8, "g %d n %d: tile %d on XCD %d", g, n, t, b % 8);
+            }
+            for (int t = 0; t < n; ++t) CHECK(seen[t] >= 0, "g %d n %d: tile %d never run", g, n, t);
+        }
+}
+
 static void check_values() {
     std::mt19937_64 g(5);
     std::uniform_real_distribution<double> u(-1.0, 1.0);
@@ -165,6 +185,7 @@ int main() {
         check_layout(natom, true);
     }
     check_values();
+    check_band_tile();
     printf("ok\n");
     return 0;
 }

@@ -1,5 +1,6 @@
-"""Lattice-state layout (IonLayout) and compensated FP32 value format
-(pk_pack / pk_unpack) of php_fdtd_amd/csrc/phpfdtd_internal.h, checked on the
+"""Lattice-state layout (IonLayout), compensated FP32 value format
+(pk_pack / pk_unpack) and Yee tile order (band_tile) of
+php_fdtd_amd/csrc/phpfdtd_internal.h, checked on the
 host by tests/layout_check.cpp, which includes the header the kernels and the
 state I/O use:
 
@@ -20,7 +21,10 @@ state I/O use:
     unit of 2^-4 ulp (2^-28 relative), one unit (2^-27) where a +-8
     correction is clamped to +-7; re-packing a stored value is the identity;
     a code read or written at a nibble index (or at a bit of a dword) is
-    sign-extended and leaves its neighbour as it was.
+    sign-extended and leaves its neighbour as it was;
+  * band_tile (work-group -> tile of the single-domain Yee launches): over a
+    launch of a multiple of 8 g work-groups every tile runs exactly once, and
+    each group of g tiles on one XCD, the groups going to the XCDs in turn.
 """
 import os
 import subprocess
