@@ -409,8 +409,9 @@ def run_reference(spec, med, nsteps, ranks, timeout, log_path=None):
 
 
 def cpu_baseline(workload="tip_slab", full_steps=1, prefix_steps=10, timeout=1200, log_path=None):
-    """The reference solver on the host cores of this box (BASELINE.md section
-    3): the workload's own grid for `full_steps` steps when host RAM holds the
+    """The reference solver on the host cores of this box, built with the
+    reference's README.md:122 flags and run under mpirun with its loop timer
+    (fdtd.c:3917): the workload's own grid for `full_steps` steps when host RAM holds the
     reference's layout of it; otherwise the workload is recorded as not
     runnable and a labelled prefix of the 384x384x256 16-atom configuration
     is timed instead.  Ranks: the CPU share of this process (the box allots
