@@ -340,16 +340,17 @@ static void plan_env(phpfdtd_ctx *x) {
     if (x->batch) return;
     const char *yp = getenv("PHPFDTD_YEE");
     x->fused = !(yp && strcmp(yp, "split") == 0);
-    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? (x->fused ? 16 : 4) : 16;
+    x->yee_kchunk = x->prec == PHPFDTD_FP32 ? (x->fused ? 12 : 4) : 16;
     if (const char *e = getenv("PHPFDTD_YEE_KCHUNK")) {
         const int v = atoi(e);
         if (v >= 1) x->yee_kchunk = v;
     }
-    // 16-plane chunks; FP32 (E-form, 127 VGPRs, 4 wavefronts per SIMD): 8
-    // rows, two work-groups per CU, 7.6-8.4 ms against 7.9-8.4 (16 rows, one
-    // work-group of 16 wavefronts) and 8.9-9.2 (12: one work-group of 12);
-    // FP64 8 rows 21.70 ms against 22.44 (4) and 23.25 (12: LDS-bound to one
-    // work-group per CU)
+    // Measured on the 1024x1024x640 case.  FP32 (E-form, 127 VGPRs, 4
+    // wavefronts per SIMD): 12-plane chunks 7.62-7.66 ms against 7.73-7.74
+    // for 16 (32: 7.91, 64: 8.29); 8 rows (two work-groups per CU) 7.7 ms
+    // against 8.1 for 16 rows (one work-group of 16 wavefronts), 9.3 for 4
+    // rows.  FP64: 16-plane chunks; 8 rows 21.70 ms against 22.44 (4) and
+    // 23.25 (12: LDS-bound to one work-group per CU) before the band order.
     x->yee_rows = 8;
     if (const char *e = getenv("PHPFDTD_YEE_ROWS")) {
         const int v = atoi(e);
