@@ -18,9 +18,9 @@
 // Two plans give the same bits:
 //   * fused (yee_update, the default): one pass.  At each plane a wavefront
 //     computes the new H of its cells, then their new D and the E of plain
-//     cells.  The four wavefronts of a work-group hold four consecutive rows
-//     and pass E and the new H of their rows to the neighbouring wavefront
-//     through LDS.  D needs the new H of neighbours outside the work-group -
+//     cells.  The R wavefronts of a work-group (R = 8 by default) hold R
+//     consecutive rows and pass E and the new H of their rows to the
+//     neighbouring wavefront through LDS.  D needs the new H of neighbours outside the work-group -
 //     row j0-1, plane k0-1 at the start of the chunk, and cell x0-1 for lane
 //     0 - which it recomputes from the old E, H and psi with the same code
 //     (h_cells) their owner runs; the old values cannot have been replaced by
@@ -28,7 +28,8 @@
 //     per cell and step: 18 words plus the material id.
 //   * split (h_update, then de_update): the new H is written, then read back
 //     by the D/E kernel: 21 words per cell.  Batched launches (many small
-//     domains in one grid) use this plan: its tiles are smaller.
+//     domains in one grid) run the one-pass tile too (yee_update_batch),
+//     or this plan on request.
 // The arithmetic is written with explicit fused multiply-adds and the file is
 // compiled without FP contraction, so the owner's H and every recomputation
 // of it round identically.
@@ -637,12 +638,12 @@ __device__ __forceinline__ TileOf tile_of(const YeeArgs<real> &a, int t) {
 }
 
 // Fused plan: H, then D and E, plane by plane, for tile t of one domain.
-// The four wavefronts of a work-group hold four consecutive rows and trade
+// The ROWS wavefronts of a work-group hold ROWS consecutive rows and trade
 // neighbour rows through LDS at one barrier per plane: a wavefront posts E of
 // plane k+1 (row j+1's H input at the next plane, for the wavefront below)
 // and its new H of plane k (row j-1's D input, for the wavefront above).
-// Wavefront 0 recomputes the new H of row j0-1 and wavefront 3 loads E of
-// row j0+4; every slot is double-buffered by plane parity.
+// Wavefront 0 recomputes the new H of row j0-1 and the last wavefront loads
+// E of row j0+ROWS; every slot is double-buffered by plane parity.
 template <typename real, int ROWS>
 struct YeeRows {
     typename Vec<real>::type ex[2][ROWS][64], ez[2][ROWS][64];  // E of plane k (parity k & 1)
